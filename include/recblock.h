@@ -1,0 +1,184 @@
+/*
+ * recblock.h -- C ABI of the MI355X (gfx950) recursive-block SpTRSV library
+ * (librecblock.so).
+ *
+ * All pointers named d_* are device pointers (hipMalloc'd, device 0 unless
+ * the caller made another device current); all others are host pointers.
+ * Indices are 0-based int32; values are IEEE double.  A matrix is square
+ * (m == n) compressed sparse column (CSC):
+ *   forward  (substitution 0): lower-triangular L, row ids ascending in each
+ *            column, so the diagonal is the first entry of every column;
+ *   backward (substitution 1): upper-triangular U, row ids ascending, the
+ *            diagonal is the last entry of every column.
+ * No function aborts: every entry point returns an rb_status (the legacy
+ * entry points, which return void as the reference's do, leave their status
+ * in rb_last_status() / rb_last_error()).
+ */
+#ifndef RECBLOCK_H
+#define RECBLOCK_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    RB_OK = 0,
+    RB_ERR_INVALID_ARG = 1,   /* null pointer, bad size, lv out of range */
+    RB_ERR_NOT_SQUARE = 2,    /* m != n */
+    RB_ERR_BAD_STRUCTURE = 3, /* colptr not monotone, row id out of range,
+                                 entry outside the triangle, unsorted or
+                                 duplicate row ids in a column */
+    RB_ERR_MISSING_DIAG = 4,  /* a column without its diagonal entry (the
+                                 reference's level analysis hangs here) */
+    RB_ERR_ZERO_DIAG = 5,     /* a zero diagonal value */
+    RB_ERR_HIP = 6,           /* a HIP runtime call failed */
+    RB_ERR_NO_MEMORY = 7,     /* device allocation failed */
+    RB_ERR_INTERNAL = 8
+} rb_status;
+
+#define RB_SUBSTITUTION_FORWARD 0
+#define RB_SUBSTITUTION_BACKWARD 1
+
+/* Execution schedules (rb_opts.schedule). */
+#define RB_SCHED_AUTO 0      /* the library's tuner picks (default) */
+#define RB_SCHED_BLOCKED 1   /* per-block kernels in the reference selector's
+                                order: TRSV, SpMV, TRSV, ... */
+#define RB_SCHED_FUSED 2     /* one persistent dataflow launch per solve */
+
+typedef struct {
+    int schedule;      /* RB_SCHED_* */
+    int device_id;     /* device whose lv auto-threshold applies (lv = -1) */
+    int reserved[6];
+} rb_opts;
+
+typedef struct rb_plan rb_plan;
+
+/* ---------------------------------------------------------------------
+ * Legacy entry points (same C signatures and argument meaning as the
+ * reference; VALUE_TYPE is double).
+ * ------------------------------------------------------------------- */
+
+/* Replaces recblocking_solver_cuda (R/recblocking_solver_cuda.h:21-32,
+ * called at R/main.cu:278).  Device CSC of L (forward) or U (backward);
+ * solves once and then BENCH_REPEAT (200, see rb_set_bench_repeat) more
+ * times; *cal_time = mean solve time in ms, *preprocess_time = plan build
+ * time in ms (set for both directions; the reference sets it for forward
+ * only).  lv >= 1 as in the reference; lv = -1 applies the auto rule of
+ * R/main.cu:245-254 for device 0.  d_x receives x in the original order.
+ * Difference: d_b is left unchanged (the reference overwrites it with the
+ * permuted, partially updated right-hand side). */
+void recblocking_solver_cuda(int *d_cscColPtrTR, int *d_cscRowIdxTR,
+                             double *d_cscValTR, int m, int n, int nnzTR,
+                             double *d_x, double *d_b, int substitution,
+                             int lv, double *cal_time,
+                             double *preprocess_time);
+
+/* Replaces recblocking_solver (R/recblocking_solver.h:16-28).  Host CSC,
+ * host b (m x rhs, column-major) and x (n x rhs).  x_ref is unused, as in
+ * the reference.  All rhs columns are solved (the reference sizes buffers
+ * for rhs but solves one).  *cal_time = mean ms per solve of all rhs. */
+void recblocking_solver(int *cscColPtr, int *cscRowIdx, double *cscVal,
+                        int m, int n, int nnz, double *b, double *x,
+                        double *x_ref, int rhs, int lv, int substitution,
+                        double *cal_time);
+
+/* Status of the last legacy call on this thread, and a message. */
+rb_status rb_last_status(void);
+const char *rb_last_error(void);
+
+/* Number of timed repeats of the legacy entry points (reference default
+ * BENCH_REPEAT = 200, R/common.h:13-15).  Returns the previous value. */
+int rb_set_bench_repeat(int repeat);
+
+/* The lv auto rule of R/main.cu:245-254. */
+int rb_lv_auto(int m, int device_id);
+
+/* ---------------------------------------------------------------------
+ * Plan API (split of the legacy calls into preprocessing and solve;
+ * the reference's internal split is mat_preprocessing / L_preprocessing /
+ * U_preprocessing then L_calculate / U_calculate,
+ * R/recblocking_preprocess.h:15,298,387,969 and
+ * R/recblocking_calculate.h:15,170).
+ * ------------------------------------------------------------------- */
+
+void rb_opts_default(rb_opts *opts);
+
+/* Build a plan from a device CSC.  Runs the level analysis, level-order
+ * permutation, recursive partition, block extraction, selector statistics
+ * and storage build on the device; the inputs are only read.  lv >= 0, or
+ * -1 for the auto rule.  stream may be 0 (the null stream).  The call
+ * synchronises on stream. */
+rb_status rb_plan_create(const int *d_colptr, const int *d_rowidx,
+                         const double *d_val, int m, int n, int nnz,
+                         int substitution, int lv, const rb_opts *opts,
+                         void *stream, rb_plan **plan);
+
+/* Solve op(T) X = B for nrhs right-hand sides: d_b is m x nrhs
+ * column-major with leading dimension ldb >= m, d_x likewise with ldx.
+ * Stream-ordered and asynchronous (no host synchronisation); d_b is not
+ * modified.  One plan must not run two solves at once; distinct plans may. */
+rb_status rb_solve(rb_plan *plan, const double *d_b, int ldb, double *d_x,
+                   int ldx, int nrhs, void *stream);
+
+rb_status rb_plan_destroy(rb_plan *plan);
+
+/* Synchronise stream and report the device-side watchdog: RB_OK, or
+ * RB_ERR_INTERNAL when a solve's bounded spin-wait gave up (it then wrote
+ * NaN instead of hanging).  No counterpart in the reference, whose spin
+ * loops have no bound. */
+rb_status rb_plan_check(rb_plan *plan, void *stream);
+
+/* Plan summary for inspection and tests. */
+typedef struct {
+    int m, nnz, substitution, lv;
+    int nblocks;          /* 2^(lv+1) - 1 */
+    int nlevel;           /* global level count */
+    int schedule;         /* schedule a solve uses (RB_SCHED_BLOCKED/FUSED) */
+    double preprocess_ms; /* plan build time */
+    size_t device_bytes;  /* device memory held by the plan */
+} rb_plan_info;
+
+rb_status rb_plan_info_get(const rb_plan *plan, rb_plan_info *info);
+
+/* Per-block selector inputs and decisions, host arrays of nblocks entries
+ * (any pointer may be NULL).  Triangular blocks are even indices, square
+ * blocks odd, in the reference's block order.  method: triangular 0..3,
+ * square 0..3 or -1 for an empty block (same meaning as the reference's
+ * SpTRSV_block.method / SpMV_block.method).  For triangular blocks nnzr =
+ * blk_nnz / blk_m; for square blocks (blk_nnz - longlen) / m_new, -1 if
+ * m_new == 0. */
+rb_status rb_plan_blocks(const rb_plan *plan, int *blk_m, int *blk_n,
+                         int *blk_nnz, int *up, int *down, int *left,
+                         int *right, int *method, int *nlv, int *nnzr,
+                         int *m_new, int *lenmax, int *longrow, int *longlen,
+                         double *empty_ratio);
+
+/* Global level analysis (host arrays): levelItem[m] is the permutation that
+ * maps level-order position i to original row levelItem[i] (reversed for
+ * backward, as the reference leaves it), levelPtr[nlevel+1] the BFS level
+ * boundaries (BFS order, before any reversal). */
+rb_status rb_plan_levels(const rb_plan *plan, int *levelItem, int *levelPtr);
+
+/* Local level analysis of triangular block t (0 <= t < 2^lv): lvptr[nlv+1],
+ * lvitem[blk_m] (block-local ids in BFS discovery order) and
+ * nnz_lv[nlv] (find_level_nnz semantics, R/utils_cuda.h:781-796). */
+rb_status rb_plan_tri_levels(const rb_plan *plan, int t, int *lvptr,
+                             int *lvitem, int *nnz_lv);
+
+/* Contents of block i as a block-local CSR (rows 0..blk_m-1, columns
+ * relative to left[i]); the diagonal of a triangular block is included.
+ * rowptr[blk_m+1], colidx[blk_nnz], val[blk_nnz] (host). */
+rb_status rb_plan_block_csr(const rb_plan *plan, int i, int *rowptr,
+                            int *colidx, double *val);
+
+/* The level-ordered (permuted) matrix as CSC, host arrays of m+1 / nnz. */
+rb_status rb_plan_permuted_csc(const rb_plan *plan, int *colptr, int *rowidx,
+                               double *val);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RECBLOCK_H */

@@ -1,0 +1,144 @@
+// Internal declarations of librecblock (gfx950).  Not installed.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/recblock.h"
+
+namespace rb {
+
+// Thread-local error reporting shared by all entry points.
+void set_error(rb_status st, const std::string &msg);
+rb_status status_of_hip(hipError_t e, const char *what);
+
+#define RB_HIP(call)                                                        \
+    do {                                                                    \
+        hipError_t _e = (call);                                             \
+        if (_e != hipSuccess) return ::rb::status_of_hip(_e, #call);        \
+    } while (0)
+
+#define RB_TRY(expr)                                                        \
+    do {                                                                    \
+        rb_status _s = (expr);                                              \
+        if (_s != RB_OK) return _s;                                         \
+    } while (0)
+
+// Sentinel bit pattern of an x slot that is not solved yet: a signalling
+// NaN payload that IEEE arithmetic never produces (results are quiet NaNs).
+constexpr uint64_t kPending = 0xFFF4A5A55A5A0001ull;
+constexpr int kLongRow = 2048;  // LONGROW_THRESHOLD, R/utils_spmv_cuda.h:9
+
+// Device allocation helper that tracks bytes held by a plan.
+struct Arena {
+    std::vector<void *> ptrs;
+    size_t bytes = 0;
+    template <class T>
+    rb_status alloc(T **p, size_t count) {
+        *p = nullptr;
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            set_error(RB_ERR_NO_MEMORY, "hipMalloc failed");
+            return RB_ERR_NO_MEMORY;
+        }
+        ptrs.push_back(*p);
+        bytes += count * sizeof(T);
+        return RB_OK;
+    }
+    void release() {
+        for (void *p : ptrs) (void)hipFree(p);
+        ptrs.clear();
+        bytes = 0;
+    }
+};
+
+// Device-wide primitives (rb_prim.hip, hipCUB).
+rb_status exclusive_scan_i32(const int *d_in, int *d_out, int n,
+                             hipStream_t s);
+rb_status inclusive_max_i32(const int *d_in, int *d_out, int n,
+                            hipStream_t s);
+rb_status sort_pairs_i32(const int *keys_in, int *keys_out,
+                         const int *vals_in, int *vals_out, int n, int bits,
+                         hipStream_t s);
+
+// Level analysis: Kahn BFS in the reference's discovery order
+// (rb_levels.hip).  colptr/rowidx: square CSC whose columns hold the
+// diagonal; rowcnt[v]: entries of row v including the diagonal.
+// Writes levelItem[m], levelPtr[nlevel+1] (device) and *nlevel (host).
+rb_status bfs_levels(const int *d_colptr, const int *d_rowidx,
+                     const int *d_rowcnt, int m, int *d_levelItem,
+                     int *d_levelPtr, int *nlevel, hipStream_t s);
+
+// Per-block descriptor kept on the host (and mirrored on the device for the
+// fused executor).
+struct Block {
+    int kind;  // 0 triangular, 1 square
+    int up, down, left, right;
+    int blk_m, blk_n, blk_nnz;
+    int slot_base;  // first slot (row) of this block in the slot space
+    int method;
+    int nlv, nnzr, m_new, lenmax, longrow, longlen;
+    double empty_ratio;
+    int dcsr_base;   // first entry in dcsr_rows
+    int long_base;   // first entry in long_rows
+    int lv_base;     // triangular: first entry in lvptr / lvitem arrays
+    int lvptr_base;
+    std::vector<int> lvptr_h;   // method 2: local level boundaries
+    std::vector<int> nnzlv_h;   // method 2: find_level_nnz
+};
+
+}  // namespace rb
+
+// The plan object.
+struct rb_plan {
+    int m = 0, nnz = 0, subst = 0, lv = 0, nblocks = 0, ntri = 0;
+    int step = 0;
+    int nlevel = 0;
+    int device = 0;
+    int schedule = RB_SCHED_BLOCKED;
+    double preprocess_ms = 0.0;
+    rb::Arena arena;
+
+    // permutation and global levels
+    int *d_levelItem = nullptr;  // final permutation for b / x
+    int *d_levelPtr = nullptr;   // BFS level boundaries (nlevel+1)
+    // permuted CSC (kept for inspection)
+    int *d_pcolptr = nullptr, *d_prowidx = nullptr;
+    double *d_pval = nullptr;
+
+    // slot space: one slot per (block, block-local row)
+    int nslots = 0;
+    int *d_rowptr = nullptr;   // nslots+1 entry offsets (off-diagonals only)
+    int *d_col = nullptr;      // block-local column of each entry
+    double *d_val = nullptr;
+    double *d_invd = nullptr;  // 1 / diagonal per permuted row
+    int *d_dcsr_rows = nullptr;
+    int *d_long_rows = nullptr;
+    int ndcsr = 0, nlong = 0;
+    // triangular local levels
+    int *d_lvptr = nullptr;    // concatenated, sum(blk_m + 1) over tri
+    int *d_lvitem = nullptr;   // concatenated local level order (m)
+    int *d_nnzlv = nullptr;    // concatenated find_level_nnz (m)
+
+    std::vector<rb::Block> blocks;
+
+    // per-solve scratch
+    double *d_bperm = nullptr;  // m
+    double *d_xperm = nullptr;  // m
+    int *d_tickets = nullptr;   // one per triangular block
+    int *d_status = nullptr;    // device watchdog flags (0 = healthy)
+    int scratch_rhs = 0;
+};
+
+namespace rb {
+rb_status build_plan(rb_plan *p, const int *d_colptr, const int *d_rowidx,
+                     const double *d_val, hipStream_t s);
+rb_status solve_blocked(rb_plan *p, const double *d_b, int ldb, double *d_x,
+                        int ldx, int nrhs, hipStream_t s);
+}  // namespace rb

@@ -1,0 +1,277 @@
+// Level-set analysis on the GPU that reproduces the reference's BFS
+// *discovery order* exactly (R/findlevel.h:7-83, R/utils_cuda.h:29-97,
+// where it runs on a single GPU thread).
+//
+// Order rule (the sequential algorithm restated): level 0 holds the rows
+// with no off-diagonal entry, ascending.  A row v joins level k+1 when the
+// last of its dependencies is visited; in the sequential walk that is its
+// dependency u* with the largest level-order position, and v lands at the
+// position sorted by (position of u*, offset of v inside column u*).
+//
+// Parallel form, one level per round inside one cooperative launch:
+//   A  every frontier node u (position p) decrements rem[v] and raises
+//      par[v] to p for every off-diagonal v in its column (atomics);
+//   B  node u counts its children: v with rem[v] == 0 and par[v] == p;
+//   S  exclusive scan of the counts in frontier order (grid-wide);
+//   C  node u writes its children, in column order, at e + offset.
+// A grid barrier separates the phases.
+#include <hip/hip_cooperative_groups.h>
+
+#include "rb_internal.h"
+
+namespace cg = cooperative_groups;
+
+namespace rb {
+
+namespace {
+
+constexpr int kBfsThreads = 256;
+
+struct BfsCtl {
+    int s, e;     // current frontier [s, e) of levelItem
+    int nlevel;   // levels emitted so far
+    int stalled;  // 1 when a round adds nothing while rows remain
+};
+
+// Exclusive scan of one value per thread across a 256-thread block.
+// Returns the exclusive prefix; *total receives the block sum.
+__device__ inline int block_excl_scan(int v, int *lds, int *total) {
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) lds[wid] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int w = 0; w < kBfsThreads / 64; w++) {
+            int t = lds[w];
+            lds[w] = run;
+            run += t;
+        }
+        lds[kBfsThreads / 64] = run;
+    }
+    __syncthreads();
+    int res = x - v + lds[wid];
+    *total = lds[kBfsThreads / 64];
+    __syncthreads();
+    return res;
+}
+
+__device__ inline int ld_relaxed(const int *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Grid-wide exclusive scan of cnt[0..len) in place (contiguous chunk per
+// block).  Returns the grand total to every thread.  Uses two grid syncs.
+__device__ int grid_excl_scan(cg::grid_group &g, int *cnt, int len,
+                              int *blocksum, int *lds) {
+    const int G = gridDim.x;
+    const int chunk = (len + G - 1) / G;
+    const int b0 = blockIdx.x * chunk;
+    const int b1 = min(len, b0 + chunk);
+    int local = 0;
+    for (int i = b0 + threadIdx.x; i < b1; i += blockDim.x) local += cnt[i];
+    int tot;
+    (void)block_excl_scan(local, lds, &tot);
+    if (threadIdx.x == 0) blocksum[blockIdx.x] = tot;
+    g.sync();
+    // every block scans the block sums redundantly (G is small)
+    int before = 0, grand = 0;
+    {
+        int run = 0;
+        for (int base = 0; base < G; base += blockDim.x) {
+            int i = base + threadIdx.x;
+            int v = i < G ? ld_relaxed(&blocksum[i]) : 0;
+            int t;
+            int ex = block_excl_scan(v, lds, &t);
+            if (i == blockIdx.x) lds[8] = run + ex;
+            run += t;
+            __syncthreads();
+        }
+        before = lds[8];
+        grand = run;
+        __syncthreads();
+    }
+    int run = before;
+    for (int base = b0; base < b1; base += blockDim.x) {
+        int i = base + threadIdx.x;
+        int v = i < b1 ? cnt[i] : 0;
+        int t;
+        int ex = block_excl_scan(v, lds, &t);
+        if (i < b1) cnt[i] = run + ex;
+        run += t;
+    }
+    g.sync();
+    return grand;
+}
+
+__global__ void __launch_bounds__(kBfsThreads)
+k_bfs(const int *__restrict__ colptr, const int *__restrict__ rowidx, int m,
+      int *rem, int *par, int *cnt, int *levelItem, int *levelPtr,
+      int *blocksum, BfsCtl *ctl) {
+    cg::grid_group g = cg::this_grid();
+    __shared__ int lds[16];
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gsz = gridDim.x * blockDim.x;
+
+    // ---- level 0: rows without off-diagonal entries, ascending ----
+    {
+        const int G = gridDim.x;
+        const int chunk = (m + G - 1) / G;
+        const int b0 = blockIdx.x * chunk;
+        const int b1 = min(m, b0 + chunk);
+        for (int i = b0 + threadIdx.x; i < b1; i += blockDim.x)
+            cnt[i] = rem[i] == 0 ? 1 : 0;
+        g.sync();
+        int total = grid_excl_scan(g, cnt, m, blocksum, lds);
+        for (int i = b0 + threadIdx.x; i < b1; i += blockDim.x)
+            if (rem[i] == 0) levelItem[cnt[i]] = i;
+        if (gtid == 0) {
+            levelPtr[0] = 0;
+            levelPtr[1] = total;
+            ctl->s = 0;
+            ctl->e = total;
+            ctl->nlevel = 1;
+            ctl->stalled = (total == 0 && m > 0) ? 1 : 0;
+        }
+        g.sync();
+    }
+
+    while (true) {
+        const int s = ld_relaxed(&ctl->s);
+        const int e = ld_relaxed(&ctl->e);
+        if (e >= m || ld_relaxed(&ctl->stalled)) break;
+        const int len = e - s;
+        // A: decrement dependants, raise their parent position
+        for (int p = s + gtid; p < e; p += gsz) {
+            const int u = levelItem[p];
+            for (int q = colptr[u]; q < colptr[u + 1]; q++) {
+                const int v = rowidx[q];
+                if (v == u) continue;
+                atomicSub(&rem[v], 1);
+                atomicMax(&par[v], p);
+            }
+        }
+        g.sync();
+        // B: count the children each frontier node discovers
+        {
+            const int G = gridDim.x;
+            const int chunk = (len + G - 1) / G;
+            const int b0 = blockIdx.x * chunk;
+            const int b1 = min(len, b0 + chunk);
+            for (int i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
+                const int p = s + i;
+                const int u = levelItem[p];
+                int c = 0;
+                for (int q = colptr[u]; q < colptr[u + 1]; q++) {
+                    const int v = rowidx[q];
+                    if (v != u && ld_relaxed(&rem[v]) == 0 &&
+                        ld_relaxed(&par[v]) == p)
+                        c++;
+                }
+                cnt[i] = c;
+            }
+        }
+        g.sync();
+        const int total = grid_excl_scan(g, cnt, len, blocksum, lds);
+        // C: emit children in column order
+        for (int i = gtid; i < len; i += gsz) {
+            const int p = s + i;
+            const int u = levelItem[p];
+            int o = e + cnt[i];
+            for (int q = colptr[u]; q < colptr[u + 1]; q++) {
+                const int v = rowidx[q];
+                if (v != u && ld_relaxed(&rem[v]) == 0 &&
+                    ld_relaxed(&par[v]) == p)
+                    levelItem[o++] = v;
+            }
+        }
+        if (gtid == 0) {
+            const int nl = ctl->nlevel;
+            levelPtr[nl + 1] = e + total;
+            ctl->nlevel = nl + 1;
+            ctl->s = e;
+            ctl->e = e + total;
+            if (total == 0) ctl->stalled = 1;
+        }
+        g.sync();
+    }
+}
+
+__global__ void k_bfs_init(const int *__restrict__ rowcnt, int m, int *rem,
+                           int *par) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) {
+        rem[i] = rowcnt[i] - 1;
+        par[i] = -1;
+    }
+}
+
+}  // namespace
+
+rb_status bfs_levels(const int *d_colptr, const int *d_rowidx,
+                     const int *d_rowcnt, int m, int *d_levelItem,
+                     int *d_levelPtr, int *nlevel, hipStream_t s) {
+    *nlevel = 0;
+    if (m == 0) {
+        int z = 0;
+        RB_HIP(hipMemcpyAsync(d_levelPtr, &z, sizeof(int),
+                              hipMemcpyHostToDevice, s));
+        RB_HIP(hipStreamSynchronize(s));
+        return RB_OK;
+    }
+    int dev = 0;
+    RB_HIP(hipGetDevice(&dev));
+    int ncu = 0;
+    RB_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
+                                 dev));
+    int per_cu = 0;
+    RB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_bfs,
+                                                        kBfsThreads, 0));
+    if (per_cu < 1) per_cu = 1;
+    int grid = ncu * (per_cu < 2 ? per_cu : 2);
+    int need = (m + kBfsThreads - 1) / kBfsThreads;
+    if (grid > need) grid = need;
+    if (grid < 1) grid = 1;
+
+    Arena tmp;
+    int *rem, *par, *cnt, *blocksum;
+    BfsCtl *ctl;
+    RB_TRY(tmp.alloc(&rem, m));
+    RB_TRY(tmp.alloc(&par, m));
+    RB_TRY(tmp.alloc(&cnt, m));
+    RB_TRY(tmp.alloc(&blocksum, grid));
+    RB_TRY(tmp.alloc(&ctl, 1));
+    k_bfs_init<<<(m + 255) / 256, 256, 0, s>>>(d_rowcnt, m, rem, par);
+    RB_HIP(hipGetLastError());
+    void *args[] = {(void *)&d_colptr, (void *)&d_rowidx, (void *)&m,
+                    (void *)&rem,      (void *)&par,      (void *)&cnt,
+                    (void *)&d_levelItem, (void *)&d_levelPtr,
+                    (void *)&blocksum, (void *)&ctl};
+    hipError_t e = hipLaunchCooperativeKernel((const void *)k_bfs, dim3(grid),
+                                              dim3(kBfsThreads), args, 0, s);
+    if (e != hipSuccess) {
+        tmp.release();
+        return status_of_hip(e, "hipLaunchCooperativeKernel(k_bfs)");
+    }
+    BfsCtl h;
+    e = hipMemcpyAsync(&h, ctl, sizeof(h), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    tmp.release();
+    if (e != hipSuccess) return status_of_hip(e, "k_bfs");
+    if (h.stalled) {
+        set_error(RB_ERR_MISSING_DIAG,
+                  "level analysis stalled: a row is never released "
+                  "(missing diagonal or cyclic dependency)");
+        return RB_ERR_MISSING_DIAG;
+    }
+    *nlevel = h.nlevel;
+    return RB_OK;
+}
+
+}  // namespace rb

@@ -1,0 +1,210 @@
+"""Host-side mirror of the reference interface over the C ABI.
+
+``recblocking_solver_cuda`` / ``recblocking_solver`` keep the reference's
+names and argument meaning (R/recblocking_solver_cuda.h:21-32,
+R/recblocking_solver.h:16-28); ``Plan`` is the split preprocessing / solve
+interface.  Device memory and streams come from torch (ROCm build); the
+compute is librecblock.so.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("no GPU visible: librecblock needs a gfx950 device")
+    return torch
+
+
+def _dev_i32(a):
+    torch = _torch()
+    return torch.as_tensor(np.ascontiguousarray(a, np.int32), device="cuda")
+
+
+def _dev_f64(a):
+    torch = _torch()
+    return torch.as_tensor(np.ascontiguousarray(a, np.float64), device="cuda")
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        return None
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+class Plan:
+    """Preprocessed factor resident in HBM.
+
+    ``colptr``/``rowidx``/``val`` are host arrays or torch device tensors of
+    a square CSC (lower for substitution 0, upper for 1).
+    """
+
+    def __init__(self, colptr, rowidx, val, substitution=0, lv=-1,
+                 schedule=_lib.SCHED_AUTO, stream=None, device_id=0):
+        torch = _torch()
+        self._keep = []
+        dcp = colptr if torch.is_tensor(colptr) else _dev_i32(colptr)
+        dri = rowidx if torch.is_tensor(rowidx) else _dev_i32(rowidx)
+        dv = val if torch.is_tensor(val) else _dev_f64(val)
+        self.m = int(dcp.numel()) - 1
+        self.nnz = int(dri.numel())
+        opts = _lib.rb_opts()
+        lib().rb_opts_default(ctypes.byref(opts))
+        opts.schedule = schedule
+        opts.device_id = device_id
+        h = ctypes.c_void_p()
+        check(lib().rb_plan_create(_ptr(dcp), _ptr(dri), _ptr(dv), self.m,
+                                   self.m, self.nnz, substitution, lv,
+                                   ctypes.byref(opts), _stream_ptr(stream),
+                                   ctypes.byref(h)))
+        self._h = h
+        self.substitution = substitution
+        info = self.info()
+        self.lv = info.lv
+        self.nblocks = info.nblocks
+        self.nlevel = info.nlevel
+
+    @classmethod
+    def from_factor(cls, f, lv=-1, **kw):
+        return cls(f.colptr, f.rowidx, f.val, f.substitution, lv, **kw)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().rb_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        inf = _lib.rb_plan_info()
+        check(lib().rb_plan_info_get(self._h, ctypes.byref(inf)))
+        return inf
+
+    # ------------------------------------------------------------ solve
+    def solve(self, b, x=None, stream=None):
+        """Solve for b (torch device tensor, shape (m,) or (m, k) with
+        column-major storage, i.e. b.t() contiguous).  Returns x."""
+        torch = _torch()
+        if not torch.is_tensor(b):
+            b = _dev_f64(b)
+        if b.dim() == 1:
+            nrhs, ldb = 1, self.m
+            if x is None:
+                x = torch.empty_like(b)
+            ldx = self.m
+        else:
+            nrhs = b.shape[1]
+            if b.stride(0) != 1:
+                raise ValueError("b must be column-major (b.t() contiguous)")
+            ldb = b.stride(1)
+            if x is None:
+                x = torch.empty((nrhs, self.m), dtype=b.dtype,
+                                device=b.device).t()
+            ldx = x.stride(1)
+        check(lib().rb_solve(self._h, _ptr(b), ldb, _ptr(x), ldx, nrhs,
+                             _stream_ptr(stream)))
+        return x
+
+    def check(self, stream=None):
+        """Synchronise and raise if the device watchdog fired."""
+        check(lib().rb_plan_check(self._h, _stream_ptr(stream)))
+
+    # ------------------------------------------------------- inspection
+    def blocks(self):
+        nb = self.nblocks
+        names = ["blk_m", "blk_n", "blk_nnz", "up", "down", "left", "right",
+                 "method", "nlv", "nnzr", "m_new", "lenmax", "longrow",
+                 "longlen"]
+        arrs = {k: np.zeros(nb, np.int32) for k in names}
+        er = np.zeros(nb, np.float64)
+        check(lib().rb_plan_blocks(self._h, *[arrs[k].ctypes.data for k in names],
+                                   er.ctypes.data))
+        arrs["empty_ratio"] = er
+        return arrs
+
+    def levels(self):
+        li = np.zeros(max(self.m, 1), np.int32)
+        lp = np.zeros(self.nlevel + 1, np.int32)
+        check(lib().rb_plan_levels(self._h, li.ctypes.data, lp.ctypes.data))
+        return li[:self.m], lp
+
+    def tri_levels(self, t):
+        bl = self.blocks()
+        bm = int(bl["blk_m"][2 * t])
+        nlv = int(bl["nlv"][2 * t])
+        lp = np.zeros(nlv + 1, np.int32)
+        li = np.zeros(max(bm, 1), np.int32)
+        nl = np.zeros(max(nlv, 1), np.int32)
+        check(lib().rb_plan_tri_levels(self._h, t, lp.ctypes.data,
+                                       li.ctypes.data, nl.ctypes.data))
+        return lp, li[:bm], nl[:nlv]
+
+    def block_csr(self, i):
+        bl = self.blocks()
+        bm, nz = int(bl["blk_m"][i]), int(bl["blk_nnz"][i])
+        rp = np.zeros(bm + 1, np.int32)
+        ci = np.zeros(max(nz, 1), np.int32)
+        vv = np.zeros(max(nz, 1), np.float64)
+        check(lib().rb_plan_block_csr(self._h, i, rp.ctypes.data,
+                                      ci.ctypes.data, vv.ctypes.data))
+        return rp, ci[:nz], vv[:nz]
+
+    def permuted_csc(self):
+        cp = np.zeros(self.m + 1, np.int32)
+        ri = np.zeros(max(self.nnz, 1), np.int32)
+        vv = np.zeros(max(self.nnz, 1), np.float64)
+        check(lib().rb_plan_permuted_csc(self._h, cp.ctypes.data,
+                                         ri.ctypes.data, vv.ctypes.data))
+        return cp, ri[:self.nnz], vv[:self.nnz]
+
+
+def recblocking_solver_cuda(d_cscColPtrTR, d_cscRowIdxTR, d_cscValTR, m, n,
+                            nnzTR, d_x, d_b, substitution, lv):
+    """Reference entry point (R/recblocking_solver_cuda.h:21-32) on torch
+    device tensors.  Returns (cal_time_ms, preprocess_time_ms); raises on a
+    failed call instead of hanging."""
+    ct, pt = ctypes.c_double(0.0), ctypes.c_double(0.0)
+    lib().recblocking_solver_cuda(_ptr(d_cscColPtrTR), _ptr(d_cscRowIdxTR),
+                                  _ptr(d_cscValTR), m, n, nnzTR, _ptr(d_x),
+                                  _ptr(d_b), substitution, lv,
+                                  ctypes.byref(ct), ctypes.byref(pt))
+    check(lib().rb_last_status())
+    return ct.value, pt.value
+
+
+def recblocking_solver(cscColPtr, cscRowIdx, cscVal, m, n, nnz, b, x, x_ref,
+                       rhs, lv, substitution):
+    """Reference host-pointer entry point (R/recblocking_solver.h:16-28) on
+    numpy arrays; x is written in place.  Returns cal_time_ms."""
+    cp = np.ascontiguousarray(cscColPtr, np.int32)
+    ri = np.ascontiguousarray(cscRowIdx, np.int32)
+    vv = np.ascontiguousarray(cscVal, np.float64)
+    bb = np.ascontiguousarray(b, np.float64)
+    assert x.dtype == np.float64 and x.flags["C_CONTIGUOUS"]
+    ct = ctypes.c_double(0.0)
+    lib().recblocking_solver(cp.ctypes.data, ri.ctypes.data, vv.ctypes.data,
+                             m, n, nnz, bb.ctypes.data, x.ctypes.data, None,
+                             rhs, lv, substitution, ctypes.byref(ct))
+    check(lib().rb_last_status())
+    return ct.value
+
+
+def set_bench_repeat(n):
+    return lib().rb_set_bench_repeat(int(n))
+
+
+def lv_auto(m, device_id=0):
+    return lib().rb_lv_auto(int(m), int(device_id))

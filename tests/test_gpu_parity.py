@@ -1,0 +1,249 @@
+"""GPU parity: the device pipeline against the CPU oracle on the same seeded
+inputs.  Integer outputs (level sets, permutation, partition, selector
+inputs and decisions, block contents) must be bit-exact; x must agree
+within relative 1e-10 (FP64), and the residual ||b - T x|| / ||b|| is
+checked against 1e-12."""
+import numpy as np
+import pytest
+
+from helpers import block_sets, csc_from_deps, rel_err, residual
+from rbsptrsv import gen
+
+pytestmark = pytest.mark.gpu
+
+X_TOL = 1e-10      # relative max error of x against the oracle (FP64)
+RES_TOL = 1e-12    # relative residual
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a device"
+    return torch
+
+
+def _plan(f, lv, **kw):
+    from rbsptrsv import Plan
+    return Plan.from_factor(f, lv=lv, **kw)
+
+
+def _check_pipeline(oracle, f, lv):
+    """Levels, permutation, partition, selector and block contents."""
+    s = f.substitution
+    re, part, bnz, an = oracle.full_pipeline(f.colptr, f.rowidx, f.val, f.n, s, lv)
+    p = _plan(f, lv)
+    li, lp = p.levels()
+    assert p.nlevel == re["nlevel"]
+    assert np.array_equal(lp, re["levelPtr"])
+    assert np.array_equal(li, re["levelItem"])
+    cp, ri, vv = p.permuted_csc()
+    assert np.array_equal(cp, re["colptr"])
+    assert np.array_equal(ri, re["rowidx"])
+    assert np.array_equal(vv, re["val"])
+    bl = p.blocks()
+    for k in ("blk_m", "blk_n", "up", "down", "left", "right"):
+        assert np.array_equal(bl[k], part[k]), k
+    assert np.array_equal(bl["blk_nnz"], bnz)
+    tri = np.arange(0, p.nblocks, 2)
+    sq = np.arange(1, p.nblocks, 2)
+    assert np.array_equal(bl["method"], an["method"])
+    assert np.array_equal(bl["nlv"][tri], an["nlv"][tri])
+    assert np.array_equal(bl["nnzr"], an["nnzr"])
+    for k in ("m_new", "lenmax", "longrow", "longlen"):
+        assert np.array_equal(bl[k][sq], an[k][sq]), k
+    assert np.array_equal(bl["empty_ratio"][sq], an["empty_ratio"][sq])
+    for t in range(len(tri)):
+        lvp, lvi, nlv = p.tri_levels(t)
+        assert np.array_equal(lvp, an["lvptr_local"][t]), t
+        assert np.array_equal(lvi, an["lvitem_local"][t]), t
+        assert np.array_equal(nlv, an["nnz_lv"][t]), t
+    # block contents as sets, against the oracle's permuted matrix
+    for i in range(p.nblocks):
+        want = block_sets(re["colptr"], re["rowidx"], re["val"], part, i, s)
+        rp, ci, v = p.block_csr(i)
+        got = {}
+        for r in range(len(rp) - 1):
+            for q in range(rp[r], rp[r + 1]):
+                got[(r, int(ci[q]))] = float(v[q])
+        assert got.keys() == want.keys(), i
+        for k in want:
+            assert got[k] == pytest.approx(want[k], rel=1e-15), (i, k)
+    return p, re, part
+
+
+SMALL = [
+    ("random300", lambda: gen.random_lower(300, 3, seed=1, window=40)),
+    ("random2000", lambda: gen.random_lower(2000, 6, seed=2)),
+    ("laplacian40x30", lambda: gen.laplacian(40, 30)),
+    ("banded20000", lambda: gen.banded(20000, seed=5)),
+    ("powerlaw30000", lambda: gen.powerlaw(30000, seed=6)),
+]
+
+
+@pytest.mark.parametrize("name,make", SMALL)
+@pytest.mark.parametrize("lower", [True, False])
+@pytest.mark.parametrize("lv", [1, 2, 4])
+def test_pipeline_bit_exact(oracle, torch, name, make, lower, lv):
+    f = make()
+    if not lower:
+        f = f.transpose()
+    _say("pipeline", name, "lower" if lower else "upper", "lv", lv)
+    _check_pipeline(oracle, f, lv)
+
+
+@pytest.mark.parametrize("name,make", SMALL)
+@pytest.mark.parametrize("lower", [True, False])
+@pytest.mark.parametrize("lv", [1, 3, -1])
+def test_solve_matches_oracle(oracle, torch, name, make, lower, lv):
+    f = make()
+    if not lower:
+        f = f.transpose()
+    _say("solve", name, "lower" if lower else "upper", "lv", lv)
+    b = gen.rhs(f.n, seed=3)
+    lvv = oracle.lv_auto(f.n) if lv == -1 else lv
+    re, part, bnz, an = oracle.full_pipeline(f.colptr, f.rowidx, f.val, f.n,
+                                             f.substitution, lvv)
+    xo = oracle.solve_blocks(re, f.n, lvv, f.substitution, part, b)
+    p = _plan(f, lv)
+    x = p.solve(torch.as_tensor(b, device="cuda")).cpu().numpy()
+    assert rel_err(x, xo) < X_TOL
+    assert residual(f, x, b) < RES_TOL
+
+
+def test_handworked_fixture_on_device(oracle, torch):
+    import json
+    import os
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                    "handworked.json")))["bfs_forward"]
+    from rbsptrsv import Plan
+    val = np.arange(1, len(g["rowidx"]) + 1, dtype=np.float64)
+    p = Plan(np.array(g["colptr"]), np.array(g["rowidx"]), val, 0, 1)
+    li, lp = p.levels()
+    assert li.tolist() == g["levelItem_final"]
+    assert lp.tolist() == g["levelPtr"]
+    cp, ri, _ = p.permuted_csc()
+    assert cp.tolist() == g["colptr_new"] and ri.tolist() == g["rowidx_new"]
+
+
+def test_multi_rhs_and_leading_dimension(oracle, torch):
+    f = gen.banded(5000, seed=8)
+    p = _plan(f, 2)
+    k, ld = 3, f.n + 17
+    B = np.random.default_rng(1).uniform(-1, 1, (k, ld))
+    Bd = torch.as_tensor(B, device="cuda").t()          # (ld, k) column-major
+    X = torch.zeros((k, ld), dtype=torch.float64, device="cuda").t()
+    from rbsptrsv._lib import check, lib
+    import ctypes
+    check(lib().rb_solve(p._h, ctypes.c_void_p(Bd.data_ptr()), ld,
+                         ctypes.c_void_p(X.data_ptr()), ld, k, None))
+    torch.cuda.synchronize()
+    Xh = X.t().cpu().numpy()
+    re, part, bnz, an = oracle.full_pipeline(f.colptr, f.rowidx, f.val, f.n, 0, 2)
+    for j in range(k):
+        xo = oracle.solve_blocks(re, f.n, 2, 0, part, B[j, :f.n].copy())
+        assert rel_err(Xh[j, :f.n], xo) < X_TOL
+        assert np.all(Xh[j, f.n:] == 0.0)   # padding rows untouched
+
+
+def test_deterministic_bitwise(torch):
+    f = gen.powerlaw(50000, seed=2)
+    p = _plan(f, 3)
+    b = torch.as_tensor(gen.rhs(f.n, 4), device="cuda")
+    x1 = p.solve(b).cpu().numpy()
+    x2 = p.solve(b).cpu().numpy()
+    assert np.array_equal(x1.view(np.int64), x2.view(np.int64))
+
+
+# ------------------------------------------------------------- edge cases
+@pytest.mark.parametrize("n,lv", [(1, 1), (3, 2), (5, 3), (7, 1), (100, 6)])
+def test_tiny_and_more_blocks_than_rows(oracle, torch, n, lv):
+    f = gen.random_lower(n, 2, seed=n)
+    _say("tiny", n, lv)
+    p, re, part = _check_pipeline(oracle, f, lv)
+    b = gen.rhs(n, 1)
+    x = p.solve(torch.as_tensor(b, device="cuda")).cpu().numpy()
+    assert residual(f, x, b) < RES_TOL
+
+
+def test_diagonal_only(oracle, torch):
+    n = 1000
+    f = csc_from_deps(n, {}, diag=2.0)
+    p, re, part = _check_pipeline(oracle, f, 3)
+    b = gen.rhs(n, 2)
+    x = p.solve(torch.as_tensor(b, device="cuda")).cpu().numpy()
+    assert np.array_equal(x, b * 0.5)
+
+
+def test_long_rows_and_empty_square_blocks(oracle, torch):
+    n = 8000
+    deps = {7000: list(range(0, 3000)), 7001: list(range(100, 2600)),
+            7002: [5]}
+    f = csc_from_deps(n, deps, diag=float(3100))
+    p, re, part = _check_pipeline(oracle, f, 2)
+    bl = p.blocks()
+    assert int(bl["longrow"].sum()) == 2
+    b = gen.rhs(n, 3)
+    x = p.solve(torch.as_tensor(b, device="cuda")).cpu().numpy()
+    assert residual(f, x, b) < RES_TOL
+
+
+def _expect_status(torch, colptr, rowidx, val, subst, code):
+    from rbsptrsv import Plan, RbError
+    with pytest.raises(RbError) as ei:
+        Plan(np.asarray(colptr), np.asarray(rowidx), np.asarray(val, float),
+             subst, 1)
+    assert ei.value.status == code
+
+
+def test_error_codes_not_hangs(torch):
+    # missing diagonal (the reference's level analysis never terminates)
+    _expect_status(torch, [0, 1, 1, 2], [0, 2], [1.0, 1.0], 0, 4)
+    # zero diagonal
+    _expect_status(torch, [0, 2, 3], [0, 1, 1], [0.0, 1.0, 1.0], 0, 5)
+    # entry above the diagonal in a lower solve
+    _expect_status(torch, [0, 1, 3], [0, 0, 1], [1.0, 1.0, 1.0], 0, 3)
+    # unsorted column
+    _expect_status(torch, [0, 3, 4, 5], [0, 2, 1, 1, 2], [1.0] * 5, 0, 3)
+    # diagonal not last in a backward solve column
+    _expect_status(torch, [0, 1, 3], [0, 1, 0], [1.0] * 3, 1, 3)
+
+
+# ------------------------------------------------ legacy entry points
+def test_legacy_recblocking_solver_cuda(oracle, torch):
+    from rbsptrsv import recblocking_solver_cuda, set_bench_repeat
+    f = gen.banded(30000, seed=4)
+    b = f.matvec(np.ones(f.n))
+    d = lambda a, t: torch.as_tensor(np.asarray(a, t), device="cuda")
+    dcp, dri, dv = d(f.colptr, np.int32), d(f.rowidx, np.int32), d(f.val, np.float64)
+    db = d(b, np.float64)
+    dx = torch.zeros(f.n, dtype=torch.float64, device="cuda")
+    old = set_bench_repeat(3)
+    try:
+        ct, pt = recblocking_solver_cuda(dcp, dri, dv, f.n, f.n, f.nnz, dx, db,
+                                         0, -1)
+    finally:
+        set_bench_repeat(old)
+    assert ct > 0 and pt > 0
+    x = dx.cpu().numpy()
+    assert np.abs(x - 1.0).max() < 1e-12          # b = L * 1  =>  x = 1
+    assert np.array_equal(db.cpu().numpy(), b)     # b left unchanged
+
+
+def test_legacy_recblocking_solver_host_multi_rhs(oracle, torch):
+    from rbsptrsv import recblocking_solver, set_bench_repeat
+    f = gen.laplacian(50, 40).transpose()          # backward solve
+    rhs = 2
+    B = np.random.default_rng(0).uniform(-1, 1, (rhs, f.n))
+    X = np.zeros((rhs, f.n))
+    old = set_bench_repeat(2)
+    try:
+        recblocking_solver(f.colptr, f.rowidx, f.val, f.n, f.n, f.nnz,
+                           B.ravel(), X.reshape(-1), None, rhs, 2, 1)
+    finally:
+        set_bench_repeat(old)
+    for j in range(rhs):
+        assert residual(f, X[j], B[j]) < RES_TOL
