@@ -145,7 +145,7 @@ def analyze(colptr, rowidx, m, lv, subst, part, blk_nnz):
     ntri = 1 << lv
     lvptr = np.zeros(m + ntri + 1, np.int32)
     lvitem = np.zeros(max(m, 1), np.int32)
-    nnzlv = np.zeros(max(m, 1), np.int32)
+    nnzlv = np.zeros(m + ntri + 1, np.int32)
     rc = lib().orc_analyze(_i32(colptr), _i32(rowidx), m, lv, subst,
                            part["blk_m"], part["blk_n"], _i32(blk_nnz),
                            part["up"], part["down"], part["left"], part["right"],
@@ -164,7 +164,7 @@ def analyze(colptr, rowidx, m, lv, subst, part, blk_nnz):
         nlv = int(ints["nlv"][2 * t])
         lp_list.append(lvptr[po:po + nlv + 1].copy())
         li_list.append(lvitem[io:io + bm].copy())
-        nl_list.append(nnzlv[io:io + nlv].copy())
+        nl_list.append(nnzlv[po:po + nlv].copy())
         po += bm + 1
         io += bm
     out["lvptr_local"] = lp_list

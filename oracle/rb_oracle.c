@@ -336,7 +336,7 @@ void orc_block_nnz(const int *colptr, const int *rowidx, int lv, int subst,
  * Concatenated per triangular block, in triangular-block order:
  *   lvptr_local  (blk_m + 1 entries per block) local levelPtr
  *   lvitem_local (blk_m entries) local levelItem (BFS discovery order)
- *   nnz_lv       (blk_m entries; first nlv used) find_level_nnz
+ *   nnz_lv       (blk_m + 1 entries; first nlv used) find_level_nnz
  *                (R/utils_cuda.h:781-796), which sums row counts over the
  *                *position* range [lvptr[li], lvptr[li+1]).
  * Returns 0, or -1 when a local level analysis stalls. */
@@ -383,11 +383,14 @@ int orc_analyze(const int *colptr, const int *rowidx, int m, int lv,
         if (i % 2 == 0) {
             int *lp = lvptr_local + lvp_off;
             int *li = lvitem_local + item_off;
-            int *nl = nnz_lv + item_off;
+            int *nl = nnz_lv + lvp_off;
             int fasttrack = bm == bz;
             if (fasttrack) {
-                /* the reference skips the analysis; one level of bm rows */
+                /* the reference skips the analysis; one level of bm rows.
+                 * nnzr is not formed on this branch by the reference; it is
+                 * reported as blk_nnz / blk_m (1), or 0 for an empty block */
                 nlv[i] = 1;
+                nnzr[i] = bm > 0 ? bz / bm : 0;
                 lp[0] = 0;
                 lp[1] = bm;
                 for (j = 0; j < bm; j++)

@@ -78,7 +78,7 @@ rb_status rb_plan_create(const int *d_colptr, const int *d_rowidx,
         set_error(RB_ERR_NOT_SQUARE, "m != n");
         return RB_ERR_NOT_SQUARE;
     }
-    if (m < 0 || nnz < m || !d_colptr || (nnz > 0 && (!d_rowidx || !d_val)) ||
+    if (m < 0 || nnz < 0 || !d_colptr || (nnz > 0 && (!d_rowidx || !d_val)) ||
         (substitution != 0 && substitution != 1)) {
         set_error(RB_ERR_INVALID_ARG,
                   "invalid argument (sizes, null pointers or substitution)");
@@ -227,8 +227,8 @@ rb_status rb_plan_tri_levels(const rb_plan *p, int t, int *lvptr,
         RB_HIP(hipMemcpy(lvitem, p->d_lvitem + b.lv_base,
                          sizeof(int) * (size_t)b.blk_m,
                          hipMemcpyDeviceToHost));
-    if (nnz_lv && b.blk_m > 0)
-        RB_HIP(hipMemcpy(nnz_lv, p->d_nnzlv + b.lv_base,
+    if (nnz_lv)
+        RB_HIP(hipMemcpy(nnz_lv, p->d_nnzlv + b.lvptr_base,
                          sizeof(int) * (size_t)b.nlv, hipMemcpyDeviceToHost));
     return ok();
 }

@@ -124,7 +124,7 @@ struct rb_plan {
     // triangular local levels
     int *d_lvptr = nullptr;    // concatenated, sum(blk_m + 1) over tri
     int *d_lvitem = nullptr;   // concatenated local level order (m)
-    int *d_nnzlv = nullptr;    // concatenated find_level_nnz (m)
+    int *d_nnzlv = nullptr;    // concatenated find_level_nnz (lvptr layout)
 
     std::vector<rb::Block> blocks;
 

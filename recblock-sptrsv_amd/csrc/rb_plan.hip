@@ -723,7 +723,7 @@ rb_status build_plan(rb_plan *p, const int *d_colptr, const int *d_rowidx,
         }
         RB_TRY(A.alloc(&p->d_lvptr, lb));
         RB_TRY(A.alloc(&p->d_lvitem, m));
-        RB_TRY(A.alloc(&p->d_nnzlv, m));
+        RB_TRY(A.alloc(&p->d_nnzlv, lb));
         int *d_triup, *d_trilvb, *d_nlv, *d_triib, *d_trisb;
         RB_TRY(tmp.alloc(&d_triup, ntri));
         RB_TRY(tmp.alloc(&d_trilvb, ntri));
@@ -764,7 +764,7 @@ rb_status build_plan(rb_plan *p, const int *d_colptr, const int *d_rowidx,
         for (int t = 0; t < ntri; t++) hnl[t] = p->blocks[2 * t].nlv;
         RB_TRY(h2d(d_nlv, hnl, s));
         k_nnzlv<<<ntri, kT, 0, s>>>(p->d_lvptr, d_trilvb, d_nlv, d_trisb,
-                                     p->d_rowptr, p->d_nnzlv, d_triib);
+                                     p->d_rowptr, p->d_nnzlv, d_trilvb);
         RB_HIP(hipGetLastError());
         RB_HIP(hipStreamSynchronize(s));
     }
@@ -790,8 +790,8 @@ rb_status build_plan(rb_plan *p, const int *d_colptr, const int *d_rowidx,
             if (b.method == 2) {
                 RB_TRY(d2h(b.lvptr_h, p->d_lvptr + b.lvptr_base,
                            (size_t)b.nlv + 1, s));
-                RB_TRY(d2h(b.nnzlv_h, p->d_nnzlv + b.lv_base, (size_t)b.nlv,
-                           s));
+                RB_TRY(d2h(b.nnzlv_h, p->d_nnzlv + b.lvptr_base,
+                           (size_t)b.nlv, s));
             }
         } else {
             b.empty_ratio =
