@@ -1,0 +1,149 @@
+"""Benchmark: plain FP64 single-RHS SpTRSV on one MI355X.
+
+Workload (default): forward solve with the lower part of the 5-point
+Laplacian on a 1500x1500 grid in natural ordering (n = 2,250,000,
+nnz = 6,747,000, 2999 level sets), b ~ U(-1,1) seeded, lv from the auto rule
+(4).  A "step" is one solve of one right-hand side with the matrix and b
+already resident in HBM.
+
+Metric: GB/s of algorithmic traffic, B = 12*nnz + 4*(m+1) + 16*m bytes per
+solve (value + index per entry, the row pointer, reading b and writing x
+once), and that as a fraction of the 8 TB/s HBM peak.
+
+Prints ONE JSON line.  --gpus above 1 is refused: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [REPO, os.path.join(REPO, "recblock-sptrsv_amd")]
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(m, nnz, nrhs=1):
+    return 12 * nnz + 4 * (m + 1) + 16 * m * nrhs
+
+
+def make_workload(name):
+    from rbsptrsv import gen
+    if name == "laplacian":
+        return gen.laplacian(1500), "laplacian-1500x1500-lower-forward"
+    if name == "laplacian-backward":
+        return gen.laplacian(1500, lower=False), "laplacian-1500x1500-upper-backward"
+    if name == "powerlaw":
+        return gen.powerlaw(4_000_000, seed=0), "powerlaw-4M-zipf2-lower-forward"
+    if name == "banded":
+        return gen.banded(100_000, seed=0), "banded-100k-w2000-lower-forward"
+    raise SystemExit("unknown workload %s" % name)
+
+
+def cpu_baseline(f, lv, budget_s):
+    """The oracle's block-ordered CPU solve (single thread) on the same
+    factor, repeated until ~budget_s of CPU work."""
+    import numpy as np
+    import oracle
+    from rbsptrsv import gen
+    re, part, bnz, an = oracle.full_pipeline(f.colptr, f.rowidx, f.val, f.n,
+                                             f.substitution, lv)
+    b = gen.rhs(f.n, seed=1)
+    t0 = time.perf_counter()
+    reps = 0
+    while True:
+        oracle.solve_blocks(re, f.n, lv, f.substitution, part, b)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or reps >= 200:
+            break
+    per = el / reps
+    return dict(value=round(algorithmic_bytes(f.n, f.nnz) / per / 1e9, 4),
+                unit="GB/s", cores=1, kind="port",
+                sample="%d block-ordered single-thread solves of the same "
+                       "workload (%.1f ms each)" % (reps, per * 1e3),
+                ms_per_solve=round(per * 1e3, 3))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="laplacian")
+    ap.add_argument("--schedule", type=int, default=0,
+                    help="0 auto, 1 blocked, 2 fused")
+    ap.add_argument("--cpu-budget", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only (this build has no multi-GPU "
+              "path); got --gpus %d" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from rbsptrsv import Plan, gen
+
+    f, wname = make_workload(args.workload)
+    b = gen.rhs(f.n, seed=1)
+    stream = torch.cuda.current_stream()
+    t0 = time.perf_counter()
+    plan = Plan.from_factor(f, lv=-1, schedule=args.schedule)
+    pre_ms = (time.perf_counter() - t0) * 1e3
+    info = plan.info()
+    db = torch.as_tensor(b, device="cuda")
+    dx = torch.empty_like(db)
+    for _ in range(args.warmup):
+        plan.solve(db, dx)
+    plan.check()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        plan.solve(db, dx)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    plan.check()
+    ms = e0.elapsed_time(e1) / args.steps
+
+    # correctness of what was timed: relative residual of the last solve
+    import numpy as np
+    x = dx.cpu().numpy()
+    res = float(np.linalg.norm(b - f.matvec(x)) / np.linalg.norm(b))
+
+    B = algorithmic_bytes(f.n, f.nnz)
+    gbs = B / (ms * 1e-3) / 1e9
+    out = {
+        "metric": "SpTRSV GB/s (algorithmic bytes, FP64, 1 RHS)",
+        "value": round(gbs, 3),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded generator)",
+        "config": {"workload": wname, "n": f.n, "nnz": f.nnz, "lv": info.lv,
+                   "levels": info.nlevel, "rhs": 1,
+                   "schedule": {1: "blocked", 2: "fused"}.get(info.schedule,
+                                                              info.schedule),
+                   "preprocess_ms": round(pre_ms, 1),
+                   "residual": res},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 3),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 5),
+                     "traffic": None,
+                     "bytes_per_solve": B},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(f, info.lv, args.cpu_budget)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

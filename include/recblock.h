@@ -173,6 +173,17 @@ rb_status rb_plan_tri_levels(const rb_plan *plan, int t, int *lvptr,
 rb_status rb_plan_block_csr(const rb_plan *plan, int i, int *rowptr,
                             int *colidx, double *val);
 
+/* Trace of the fused schedule's last solve, recorded when the environment
+ * variable RB_FUSED_TRACE is set at plan build: 16 counters per consumer
+ * wave (wall-clock ticks of 10 ns): consumer waiting for records, waiting
+ * for global dependencies, total, steps; feeder waiting for ring space,
+ * waiting for its loads, chunks; resolver waiting for FIFO space, polling,
+ * batches, re-poll rounds.  *nwaves receives the wave count (0 when the
+ * plan has no fused schedule).  counters may be NULL.  No counterpart in
+ * the reference (which times whole solves only). */
+rb_status rb_plan_trace(const rb_plan *plan, unsigned long long *counters,
+                        int *nwaves);
+
 /* The level-ordered (permuted) matrix as CSC, host arrays of m+1 / nnz. */
 rb_status rb_plan_permuted_csc(const rb_plan *plan, int *colptr, int *rowidx,
                                double *val);

@@ -41,6 +41,70 @@ static int lv_auto(int m, int device_id) {
     return li;
 }
 
+// Schedule choice.  BLOCKED and FUSED are taken as asked; AUTO times one
+// warm solve of each candidate on a scratch right-hand side and keeps the
+// faster (the plan is built for both).
+static rb_status choose_schedule(rb_plan *p, hipStream_t s) {
+    if (p->want_schedule == RB_SCHED_BLOCKED || !p->fused) {
+        p->schedule = RB_SCHED_BLOCKED;
+        return RB_OK;
+    }
+    if (p->want_schedule == RB_SCHED_FUSED) {
+        p->schedule = RB_SCHED_FUSED;
+        return RB_OK;
+    }
+    const int m = p->m;
+    double *db = nullptr, *dx = nullptr;
+    RB_HIP(hipMalloc(&db, sizeof(double) * (size_t)m));
+    if (hipMalloc(&dx, sizeof(double) * (size_t)m) != hipSuccess) {
+        (void)hipFree(db);
+        return status_of_hip(hipGetLastError(), "hipMalloc(tune)");
+    }
+    std::vector<double> ones((size_t)m, 1.0);
+    rb_status st = RB_OK;
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    if (hipMemcpy(db, ones.data(), sizeof(double) * m, hipMemcpyHostToDevice) !=
+        hipSuccess)
+        st = RB_ERR_HIP;
+    double ms[2] = {0.0, 0.0};
+    for (int c = 0; c < 2 && st == RB_OK; c++) {
+        auto run = [&]() {
+            return c == 0 ? solve_blocked(p, db, m, dx, m, 1, s)
+                          : solve_fused(p, db, m, dx, m, 1, s);
+        };
+        st = run();  // warm
+        if (st != RB_OK) break;
+        (void)hipEventRecord(e0, s);
+        for (int r = 0; r < 3 && st == RB_OK; r++) st = run();
+        (void)hipEventRecord(e1, s);
+        (void)hipEventSynchronize(e1);
+        float t = 0.f;
+        (void)hipEventElapsedTime(&t, e0, e1);
+        ms[c] = t / 3.0;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(db);
+    (void)hipFree(dx);
+    int dstat = 0;
+    if (hipMemcpy(&dstat, p->d_status, sizeof(int), hipMemcpyDeviceToHost) !=
+        hipSuccess)
+        st = RB_ERR_HIP;
+    if (st != RB_OK) return st;
+    p->tune_ms_blocked = ms[0];
+    p->tune_ms_fused = ms[1];
+    // a fused solve that tripped the watchdog is never chosen
+    p->schedule = (dstat == 0 && ms[1] < ms[0]) ? RB_SCHED_FUSED
+                                                : RB_SCHED_BLOCKED;
+    if (dstat != 0) {
+        int z = 0;
+        (void)hipMemcpy(p->d_status, &z, sizeof(int), hipMemcpyHostToDevice);
+    }
+    return RB_OK;
+}
+
 }  // namespace rb
 
 using namespace rb;
@@ -99,10 +163,14 @@ rb_status rb_plan_create(const int *d_colptr, const int *d_rowidx,
     p->lv = lv;
     (void)hipGetDevice(&p->device);
     p->schedule = RB_SCHED_BLOCKED;
+    p->want_schedule = o.schedule;
+    p->fused_waves = o.reserved[0];
     rb_status st = build_plan(p, d_colptr, d_rowidx, d_val,
                               (hipStream_t)stream);
+    if (st == RB_OK) st = choose_schedule(p, (hipStream_t)stream);
     if (st != RB_OK) {
         (void)hipStreamSynchronize((hipStream_t)stream);
+        fused_destroy(p);
         p->arena.release();
         delete p;
         return st;
@@ -119,8 +187,11 @@ rb_status rb_solve(rb_plan *p, const double *d_b, int ldb, double *d_x,
         return RB_ERR_INVALID_ARG;
     }
     if (nrhs == 0) return ok();
-    rb_status st = solve_blocked(p, d_b, ldb, d_x, ldx, nrhs,
-                                 (hipStream_t)stream);
+    rb_status st = p->schedule == RB_SCHED_FUSED
+                       ? solve_fused(p, d_b, ldb, d_x, ldx, nrhs,
+                                     (hipStream_t)stream)
+                       : solve_blocked(p, d_b, ldb, d_x, ldx, nrhs,
+                                       (hipStream_t)stream);
     if (st != RB_OK) return st;
     return ok();
 }
@@ -145,6 +216,7 @@ rb_status rb_plan_check(rb_plan *p, void *stream) {
 rb_status rb_plan_destroy(rb_plan *p) {
     if (!p) return ok();
     (void)hipDeviceSynchronize();
+    fused_destroy(p);
     p->arena.release();
     delete p;
     return ok();

@@ -93,6 +93,8 @@ struct Block {
     std::vector<int> nnzlv_h;   // method 2: find_level_nnz
 };
 
+struct FusedPlan;
+
 }  // namespace rb
 
 // The plan object.
@@ -134,6 +136,12 @@ struct rb_plan {
     int *d_tickets = nullptr;   // one per triangular block
     int *d_status = nullptr;    // device watchdog flags (0 = healthy)
     int scratch_rhs = 0;
+
+    // fused schedule (built when requested or for the tuner)
+    rb::FusedPlan *fused = nullptr;
+    int want_schedule = RB_SCHED_AUTO;
+    int fused_waves = 0;        // wave cap for the fused partition (0 = auto)
+    double tune_ms_blocked = 0.0, tune_ms_fused = 0.0;
 };
 
 namespace rb {
@@ -141,4 +149,11 @@ rb_status build_plan(rb_plan *p, const int *d_colptr, const int *d_rowidx,
                      const double *d_val, hipStream_t s);
 rb_status solve_blocked(rb_plan *p, const double *d_b, int ldb, double *d_x,
                         int ldx, int nrhs, hipStream_t s);
+// fused schedule (rb_fused.hip)
+rb_status build_fused(rb_plan *p, const int *d_prptr, const int *d_ccol,
+                      const double *d_cval, int Wmax, hipStream_t s);
+rb_status solve_fused(rb_plan *p, const double *d_b, int ldb, double *d_x,
+                      int ldx, int nrhs, hipStream_t s);
+void fused_destroy(rb_plan *p);
+int fused_max_waves();
 }  // namespace rb

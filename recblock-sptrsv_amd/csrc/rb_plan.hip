@@ -349,23 +349,21 @@ __global__ void k_local_levels(Geom g, const int *__restrict__ item_sorted,
     const int v = item_sorted[i];
     const int t = block_of(g, v, v) >> 1;
     const int up = tri_up[t];
-    // tri blocks partition the rows; block t's segment starts at the number
-    // of rows before it, which is its up bound (forward) or m - down
-    // (backward); use the sorted order instead: the segment start is the
-    // first index whose block is t.
-    const int seg = g.subst == 0 ? up : 0;
-    (void)seg;
     const int L = lvl_of[v];
     lvitem[i] = v - up;
     const int prev_t = i > 0 ? (block_of(g, item_sorted[i - 1], item_sorted[i - 1]) >> 1) : -1;
     const int prevL = i > 0 ? lvl_of[item_sorted[i - 1]] : -1;
     (void)ntri;
     if (prev_t != t || prevL != L) {
-        // segment start of block t is stored by k_seg_start beforehand in
-        // lvptr[tri_lvbase[t] + L] as an offset; write the raw index here
+        // first element of level L of block t: the raw index into the
+        // sorted array (rebased to the block's segment on the host)
         lvptr[tri_lvbase[t] + L] = i;
     }
-    atomicMax(&nlv[t], L + 1);
+    // levels are non-decreasing inside a block's segment: its last element
+    // carries the block's level count
+    const int next_t =
+        i + 1 < g.m ? (block_of(g, item_sorted[i + 1], item_sorted[i + 1]) >> 1) : -1;
+    if (next_t != t) nlv[t] = L + 1;
 }
 
 // nnz_lv of find_level_nnz (R/utils_cuda.h:781-796): sum of row counts over
@@ -814,7 +812,11 @@ rb_status build_plan(rb_plan *p, const int *d_colptr, const int *d_rowidx,
         }
     }
 
-    // ---- 9. solve scratch
+    // ---- 9. fused schedule storage (chain-tiled partition of the rows)
+    if (p->want_schedule != RB_SCHED_BLOCKED && m > 0)
+        RB_TRY(build_fused(p, prptr, ccol, cval, p->fused_waves, s));
+
+    // ---- 10. solve scratch
     RB_TRY(A.alloc(&p->d_bperm, m));
     RB_TRY(A.alloc(&p->d_xperm, m));
     RB_TRY(A.alloc(&p->d_tickets, ntri));

@@ -19,7 +19,7 @@ EXPORTS = [
     "rb_plan_create", "rb_solve", "rb_plan_destroy", "rb_plan_check",
     "rb_plan_info_get",
     "rb_plan_blocks", "rb_plan_levels", "rb_plan_tri_levels",
-    "rb_plan_block_csr", "rb_plan_permuted_csc",
+    "rb_plan_block_csr", "rb_plan_permuted_csc", "rb_plan_trace",
 ]
 
 RB_OK = 0
@@ -99,6 +99,8 @@ def lib():
     L.rb_plan_block_csr.restype = i
     L.rb_plan_permuted_csc.argtypes = [vp, vp, vp, vp]
     L.rb_plan_permuted_csc.restype = i
+    L.rb_plan_trace.argtypes = [vp, vp, ctypes.POINTER(i)]
+    L.rb_plan_trace.restype = i
     L.recblocking_solver_cuda.argtypes = [vp, vp, vp, i, i, i, vp, vp, i, i,
                                           ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_double)]

@@ -162,6 +162,17 @@ class Plan:
                                       ci.ctypes.data, vv.ctypes.data))
         return rp, ci[:nz], vv[:nz]
 
+    def trace(self):
+        """Fused-schedule counters of the last solve, shape (waves, 16), or
+        None (see rb_plan_trace)."""
+        nw = ctypes.c_int(0)
+        check(lib().rb_plan_trace(self._h, None, ctypes.byref(nw)))
+        if nw.value == 0:
+            return None
+        out = np.zeros((nw.value, 16), np.uint64)
+        check(lib().rb_plan_trace(self._h, out.ctypes.data, ctypes.byref(nw)))
+        return out
+
     def permuted_csc(self):
         cp = np.zeros(self.m + 1, np.int32)
         ri = np.zeros(max(self.nnz, 1), np.int32)

@@ -98,18 +98,22 @@ def test_pipeline_bit_exact(oracle, torch, name, make, lower, lv):
 @pytest.mark.parametrize("name,make", SMALL)
 @pytest.mark.parametrize("lower", [True, False])
 @pytest.mark.parametrize("lv", [1, 3, -1])
-def test_solve_matches_oracle(oracle, torch, name, make, lower, lv):
+@pytest.mark.parametrize("schedule", [1, 2])
+def test_solve_matches_oracle(oracle, torch, name, make, lower, lv, schedule):
     f = make()
     if not lower:
         f = f.transpose()
-    _say("solve", name, "lower" if lower else "upper", "lv", lv)
+    _say("solve", name, "lower" if lower else "upper", "lv", lv,
+         "schedule", schedule)
     b = gen.rhs(f.n, seed=3)
     lvv = oracle.lv_auto(f.n) if lv == -1 else lv
     re, part, bnz, an = oracle.full_pipeline(f.colptr, f.rowidx, f.val, f.n,
                                              f.substitution, lvv)
     xo = oracle.solve_blocks(re, f.n, lvv, f.substitution, part, b)
-    p = _plan(f, lv)
+    p = _plan(f, lv, schedule=schedule)
+    assert p.info().schedule == schedule
     x = p.solve(torch.as_tensor(b, device="cuda")).cpu().numpy()
+    p.check()
     assert rel_err(x, xo) < X_TOL
     assert residual(f, x, b) < RES_TOL
 
