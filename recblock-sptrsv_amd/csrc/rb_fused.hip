@@ -42,7 +42,8 @@ namespace {
 typedef __attribute__((address_space(3))) void lds_void;
 
 constexpr int kR = 32;          // LDS x ring depth (steps)
-constexpr int kFF = 256;        // FIFO capacity (groups)
+constexpr int kFF = 512;        // FIFO capacity (groups)
+constexpr int kU = 4;           // resolver batches of 64 polled together
 constexpr int kH = 4;           // steps per feeder chunk
 constexpr unsigned kSpin = 1u << 22;
 constexpr int kZeroSlot = kR * 64;
@@ -112,6 +113,61 @@ struct FusedArgs {
 
 __device__ inline unsigned long long now() { return wall_clock64(); }
 
+// source codes of a record entry (non-negative: LDS ring slot)
+constexpr int kPrevSame = -1;   // previous step, same lane (register)
+constexpr int kPrevLower = -2;  // previous step, lane - 1 (DPP)
+constexpr int kPrevUpper = -3;  // previous step, lane + 1 (DPP)
+constexpr int kPad = -4;        // padding, contributes nothing
+constexpr int kPublish = 1 << 30;  // hg flag: other waves read this x
+
+// lane l receives lane l-1's value (lane 0 receives 0)
+__device__ inline double dpp_from_lower(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x138, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x138, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+// lane l receives lane l+1's value (lane 63 receives 0)
+__device__ inline double dpp_from_upper(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x130, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x130, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+
+// One lane's step record, read from an LDS ring slot (ops of 16 bytes).
+template <int EC>
+struct Rec {
+    int q, orig, hg;
+    double invd, b;
+    int src[EC];
+    double val[EC];
+    __device__ void load(const int4 (*slot)[64], int lane) {
+        using L = Layout<EC>;
+        const int4 o0 = slot[0][lane];
+        q = o0.x;
+        orig = o0.y;
+        invd = __hiloint2double(o0.w, o0.z);
+#pragma unroll
+        for (int k = 0; k < L::kSrcOps; k++) {
+            const int4 o = slot[1 + k][lane];
+            const int vv[4] = {o.x, o.y, o.z, o.w};
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int idx = k * 4 + t - 1;
+                if (k == 0 && t == 0) hg = vv[0];
+                else if (idx < EC) src[idx] = vv[t];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < L::kValOps; k++) {
+            const int4 o = slot[1 + L::kSrcOps + k][lane];
+            val[2 * k] = __hiloint2double(o.y, o.x);
+            if (2 * k + 1 < EC) val[2 * k + 1] = __hiloint2double(o.w, o.z);
+        }
+        const int4 ob = slot[L::kOps][lane];
+        b = __hiloint2double(ob.y, ob.x);
+    }
+};
+
 struct Shared {
     int rec_ready[64];  // per feeder chunk: chunk index + 1 once loaded
     int cons;           // steps consumed
@@ -152,80 +208,77 @@ k_fused(FusedArgs a) {
     if (wave == 0) {
         // ------------------------------------------------------- consumer
         int fifo_pos = 0;
+        int rec_upto = 0;     // steps whose records are known to be in LDS
+        int fifo_avail = 0;   // groups known to be resolved
+        double xprev = 0.0;   // this lane's x of the previous step
         unsigned long long t_rec = 0, t_fifo = 0, t_start = now();
-        for (int s = 0; s < ns; s++) {
-            const int c = s / kH;
-            unsigned spins = 0;
+        // wait until the records of step s are in the ring (checks whole
+        // chunks, so most steps pay no LDS poll)
+        auto wait_rec = [&](int s) -> bool {
+            if (s < rec_upto) return true;
             unsigned long long tw = a.trace ? now() : 0;
-            while (lds_load(&sh.rec_ready[c % 64]) != c + 1) {
-                if (++spins > kSpin || lds_load(&sh.abort)) break;
+            unsigned spins = 0;
+            while (true) {
+                const int c = rec_upto / kH;
+                if (lds_load(&sh.rec_ready[c % 64]) == c + 1) {
+                    rec_upto += kH;
+                    if (s < rec_upto) break;
+                    continue;
+                }
+                if (++spins > kSpin || lds_load(&sh.abort)) return false;
                 __builtin_amdgcn_s_sleep(1);
             }
-            if (spins > kSpin || lds_load(&sh.abort)) {
-                if (lane == 0) {
-                    atomicOr(a.status, 2);
-                    lds_store(&sh.abort, 1);
-                }
-                break;
-            }
-            if (a.trace) t_rec += now() - tw;
             wg_acquire();
-            const int slot = s % P;
-            const int4 o0 = ring[slot][0][lane];
-            const int q = o0.x, orig = o0.y;
-            const double invd = __hiloint2double(o0.w, o0.z);
-            int src[EC];
-            double val[EC];
-            int hg = 0;
-#pragma unroll
-            for (int k = 0; k < L::kSrcOps; k++) {
-                const int4 o = ring[slot][1 + k][lane];
-                const int vv[4] = {o.x, o.y, o.z, o.w};
-#pragma unroll
-                for (int t = 0; t < 4; t++) {
-                    const int idx = k * 4 + t - 1;
-                    if (k == 0 && t == 0) hg = vv[0];
-                    else if (idx < EC) src[idx] = vv[t];
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < L::kValOps; k++) {
-                const int4 o = ring[slot][1 + L::kSrcOps + k][lane];
-                val[2 * k] = __hiloint2double(o.y, o.x);
-                if (2 * k + 1 < EC) val[2 * k + 1] = __hiloint2double(o.w, o.z);
-            }
-            const int4 ob = ring[slot][K][lane];
-            const double bv = __hiloint2double(ob.y, ob.x);
-            // global part: one FIFO group per lane with hg != 0
-            const int has = hg & 1;
-            const int gofs = hg >> 1;
+            if (a.trace) t_rec += now() - tw;
+            return true;
+        };
+        Rec<EC> cur, nxt;
+        bool ok = ns > 0 ? wait_rec(0) : true;
+        if (ns > 0 && ok) cur.load(ring[0], lane);
+        for (int s = 0; s < ns && ok; s++) {
+            // prefetch the next step's record while this one computes
+            const bool pre = s + 1 < ns && s + 1 < rec_upto;
+            if (pre) nxt.load(ring[(s + 1) % P], lane);
+            const int has = cur.hg & 1;
+            const int gofs = (cur.hg >> 1) & 0x7fff;
             const int gtot = __shfl(gofs + has, 63, 64);
             double acc = 0.0;
             if (gtot > 0) {
-                spins = 0;
-                if (a.trace) tw = now();
-                while (lds_load(&sh.fifo_ready) < fifo_pos + gtot) {
-                    if (++spins > kSpin || lds_load(&sh.abort)) break;
-                    __builtin_amdgcn_s_sleep(1);
-                }
-                wg_acquire();
-                if (spins > kSpin || lds_load(&sh.abort)) {
-                    if (lane == 0) {
-                        atomicOr(a.status, 2);
-                        lds_store(&sh.abort, 1);
+                if (fifo_avail < fifo_pos + gtot) {
+                    unsigned long long tw = a.trace ? now() : 0;
+                    unsigned spins = 0;
+                    while ((fifo_avail = lds_load(&sh.fifo_ready)) <
+                           fifo_pos + gtot) {
+                        if (++spins > kSpin || lds_load(&sh.abort)) break;
+                        __builtin_amdgcn_s_sleep(1);
                     }
-                    break;
+                    if (spins > kSpin || lds_load(&sh.abort)) {
+                        ok = false;
+                        break;
+                    }
+                    wg_acquire();
+                    if (a.trace) t_fifo += now() - tw;
                 }
-                if (a.trace) t_fifo += now() - tw;
                 if (has) acc = fifo[(fifo_pos + gofs) % kFF];
             }
+            // previous step's x of this lane and of its neighbours
+            const double xl = dpp_from_lower(xprev), xr = dpp_from_upper(xprev);
 #pragma unroll
-            for (int j = 0; j < EC; j++) acc += val[j] * xring[src[j]];
-            const double xv = (bv - acc) * invd;
+            for (int j = 0; j < EC; j++) {
+                const int sj = cur.src[j];
+                double xv;
+                if (sj >= 0) xv = xring[sj];
+                else xv = sj == kPrevSame ? xprev
+                        : sj == kPrevLower ? xl
+                        : sj == kPrevUpper ? xr : 0.0;
+                acc += cur.val[j] * xv;
+            }
+            const double xv = (cur.b - acc) * cur.invd;
+            xprev = xv;
             xring[(s % kR) * 64 + lane] = xv;
-            if (q >= 0) {
-                st_x(&a.xq[q], xv);
-                a.x[orig] = xv;
+            if (cur.q >= 0) {
+                if (cur.hg & kPublish) st_x(&a.xq[cur.q], xv);
+                a.x[cur.orig] = xv;
             }
             fifo_pos += gtot;
             wg_release();
@@ -233,6 +286,20 @@ k_fused(FusedArgs a) {
                 lds_store(&sh.fifo_cons, fifo_pos);
                 lds_store(&sh.cons, s + 1);
             }
+            if (s + 1 < ns) {
+                if (!pre) {
+                    if (!wait_rec(s + 1)) {
+                        ok = false;
+                        break;
+                    }
+                    nxt.load(ring[(s + 1) % P], lane);
+                }
+                cur = nxt;
+            }
+        }
+        if (!ok && lane == 0) {
+            atomicOr(a.status, 2);
+            lds_store(&sh.abort, 1);
         }
         if (lane == 0) lds_store(&sh.cons, 1 << 30);
         if (a.trace && lane == 0) {
@@ -295,31 +362,53 @@ k_fused(FusedArgs a) {
         int carry_g = -1;
         int done_groups = 0;
         unsigned long long t_rspace = 0, t_poll = 0, nb = 0, nround = 0, tw = 0;
-        for (int k0 = e0; k0 < e1; k0 += 64) {
-            nb++;
+        for (int kb = e0; kb < e1; kb += 64 * kU) {
+            // metadata and first polls of kU batches of 64 entries at once,
+            // so kU * 64 loads are in flight per round trip
             if (a.trace) tw = now();
-            const int n = min(64, e1 - k0);
-            const bool act = lane < n;
-            int src = 0, gm = 0;
-            double gv = 0.0;
-            if (act) {
-                src = a.gsrc[k0 + lane];
-                gv = a.gval[k0 + lane];
-                gm = a.gmeta[k0 + lane];
+            int srcu[kU], gmu[kU];
+            double gvu[kU], xvu[kU];
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                const int idx = kb + 64 * u + lane;
+                srcu[u] = 0;
+                gmu[u] = 0;
+                gvu[u] = 0.0;
+                if (idx < e1) {
+                    srcu[u] = a.gsrc[idx];
+                    gvu[u] = a.gval[idx];
+                    gmu[u] = a.gmeta[idx];
+                }
             }
-            const int g = gm >> 1;
-            // FIFO space for every group this batch may complete
-            const int gmax = __shfl(g, n - 1, 64);
-            unsigned spins = 0;
-            while (gmax - lds_load(&sh.fifo_cons) >= kFF) {
-                if (++spins > kSpin || lds_load(&sh.abort)) break;
-                __builtin_amdgcn_s_sleep(1);
+            {
+                // FIFO space for every group these batches may complete
+                const int nt = min(64 * kU, e1 - kb);
+                const int gmax = a.gmeta[kb + nt - 1] >> 1;
+                unsigned spins = 0;
+                while (gmax - lds_load(&sh.fifo_cons) >= kFF) {
+                    if (++spins > kSpin || lds_load(&sh.abort)) break;
+                    __builtin_amdgcn_s_sleep(1);
+                }
             }
             if (a.trace) {
                 t_rspace += now() - tw;
                 tw = now();
             }
-            double xv = act ? ld_x(&a.xq[src]) : 0.0;
+#pragma unroll
+            for (int u = 0; u < kU; u++)
+                xvu[u] = kb + 64 * u + lane < e1 ? ld_x(&a.xq[srcu[u]]) : 0.0;
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+            const int k0 = kb + 64 * u;
+            if (k0 >= e1) break;
+            nb++;
+            const int n = min(64, e1 - k0);
+            const bool act = lane < n;
+            const int src = srcu[u], gm = gmu[u];
+            const double gv = gvu[u];
+            double xv = xvu[u];
+            const int g = gm >> 1;
+            unsigned spins = 0;
             bool pend = act && pending(xv);
             rtmp[lane] = act && !pend ? gv * xv : 0.0;
             gtmp[lane] = act ? gm : -2;
@@ -329,7 +418,6 @@ k_fused(FusedArgs a) {
             // later step never holds back an earlier step (that could close
             // a wait cycle between waves).
             int done_lo = 0;  // lanes below are summed and published
-            spins = 0;
             while (true) {
                 const unsigned long long pm = __ballot(pend);
                 const int fp = pm ? (int)__ffsll((long long)pm) - 1 : n;
@@ -384,6 +472,8 @@ k_fused(FusedArgs a) {
                     pend = pending(xv);
                     if (!pend) rtmp[lane] = gv * xv;
                 }
+            }
+            if (lds_load(&sh.abort)) break;
             }
             if (a.trace) t_poll += now() - tw;
             if (lds_load(&sh.abort)) break;
@@ -582,6 +672,7 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
     std::vector<int> origv((size_t)NS * 64, -1);
     std::vector<int> gsrc, gmeta;
     std::vector<double> gval;
+    std::vector<char> needpub(m, 0);
     for (int w = 0; w < W; w++) {
         gent0[w] = (int)gsrc.size();
         int gord = 0;
@@ -596,7 +687,7 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
                 int srcs[8];
                 double vals[8];
                 for (int j = 0; j < 8; j++) {
-                    srcs[j] = kZeroSlot;
+                    srcs[j] = kPad;
                     vals[j] = 0.0;
                 }
                 int hg = gofs << 1;
@@ -611,10 +702,15 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
                         const int u = qof(cc[e]);
                         const bool loc = wave[u] == w && si - stepw[u] <= kR;
                         if (loc && nl < EC) {
-                            srcs[nl] = (stepw[u] % kR) * 64 + lanew[u];
+                            const int du = lanew[u] - l;
+                            if (stepw[u] == si - 1 && du == 0) srcs[nl] = kPrevSame;
+                            else if (stepw[u] == si - 1 && du == -1) srcs[nl] = kPrevLower;
+                            else if (stepw[u] == si - 1 && du == 1) srcs[nl] = kPrevUpper;
+                            else srcs[nl] = (stepw[u] % kR) * 64 + lanew[u];
                             vals[nl] = cv[e];
                             nl++;
                         } else {
+                            needpub[u] = 1;
                             gsrc.push_back(u);
                             gval.push_back(cv[e]);
                             gmeta.push_back(gord * 2);
@@ -638,7 +734,7 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
                 // sources
                 int flat[12];
                 flat[0] = hg;
-                for (int j = 0; j < 11; j++) flat[1 + j] = j < EC ? srcs[j] : kZeroSlot;
+                for (int j = 0; j < 11; j++) flat[1 + j] = j < EC ? srcs[j] : kPad;
                 for (int k = 0; k < KS; k++)
                     op[(1 + k) * 64 + l] = make_int4(flat[4 * k], flat[4 * k + 1],
                                                      flat[4 * k + 2], flat[4 * k + 3]);
@@ -653,6 +749,11 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
             }
         }
     }
+    // rows read by other waves (or beyond the LDS window) publish their x
+    for (int q = 0; q < m; q++)
+        if (needpub[q])
+            rec[((size_t)(step0[wave[q]] + stepw[q]) * K + 1) * 64 + lanew[q]].x |=
+                kPublish;
     gent0[W] = (int)gsrc.size();
     if (gsrc.empty()) {
         gsrc.push_back(0);

@@ -110,6 +110,92 @@ __device__ int grid_excl_scan(cg::grid_group &g, int *cnt, int len,
     return grand;
 }
 
+constexpr int kNarrow = 4096;  // frontier size handled by one workgroup
+
+// Consecutive narrow levels inside workgroup 0: the same A / B / scan / C
+// phases, separated by workgroup barriers.  Frontier counts live in LDS.
+// rem / par / levelItem are read with agent-scope loads (they are updated
+// by device-scope atomics and plain stores from other waves).
+__device__ void bfs_narrow(const int *__restrict__ colptr,
+                           const int *__restrict__ rowidx, int m, int *rem,
+                           int *par, int *levelItem, int *levelPtr,
+                           BfsCtl *ctl, int *lds) {
+    __shared__ int cl[kNarrow];
+    const int tid = threadIdx.x;
+    int s = ld_relaxed(&ctl->s), e = ld_relaxed(&ctl->e);
+    int nl = ld_relaxed(&ctl->nlevel);
+    int stalled = 0;
+    while (e < m && e - s <= kNarrow) {
+        const int len = e - s;
+        for (int p = s + tid; p < e; p += blockDim.x) {
+            const int u = ld_relaxed(&levelItem[p]);
+            for (int q = colptr[u]; q < colptr[u + 1]; q++) {
+                const int v = rowidx[q];
+                if (v == u) continue;
+                atomicSub(&rem[v], 1);
+                atomicMax(&par[v], p);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        for (int i = tid; i < len; i += blockDim.x) {
+            const int p = s + i;
+            const int u = ld_relaxed(&levelItem[p]);
+            int c = 0;
+            for (int q = colptr[u]; q < colptr[u + 1]; q++) {
+                const int v = rowidx[q];
+                if (v != u && ld_relaxed(&rem[v]) == 0 &&
+                    ld_relaxed(&par[v]) == p)
+                    c++;
+            }
+            cl[i] = c;
+        }
+        __syncthreads();
+        const int per = (len + blockDim.x - 1) / blockDim.x;
+        const int lo = min(len, tid * per), hi = min(len, lo + per);
+        int local = 0;
+        for (int i = lo; i < hi; i++) local += cl[i];
+        int total;
+        int run = block_excl_scan(local, lds, &total);
+        for (int i = lo; i < hi; i++) {
+            const int t = cl[i];
+            cl[i] = run;
+            run += t;
+        }
+        __syncthreads();
+        for (int i = tid; i < len; i += blockDim.x) {
+            const int p = s + i;
+            const int u = ld_relaxed(&levelItem[p]);
+            int o = e + cl[i];
+            for (int q = colptr[u]; q < colptr[u + 1]; q++) {
+                const int v = rowidx[q];
+                if (v != u && ld_relaxed(&rem[v]) == 0 &&
+                    ld_relaxed(&par[v]) == p)
+                    __hip_atomic_store(&levelItem[o++], v, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        if (tid == 0) levelPtr[nl + 1] = e + total;
+        nl++;
+        s = e;
+        e += total;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        if (total == 0) {
+            stalled = 1;
+            break;
+        }
+    }
+    if (tid == 0) {
+        ctl->s = s;
+        ctl->e = e;
+        ctl->nlevel = nl;
+        if (stalled) ctl->stalled = 1;
+    }
+}
+
 __global__ void __launch_bounds__(kBfsThreads)
 k_bfs(const int *__restrict__ colptr, const int *__restrict__ rowidx, int m,
       int *rem, int *par, int *cnt, int *levelItem, int *levelPtr,
@@ -147,6 +233,15 @@ k_bfs(const int *__restrict__ colptr, const int *__restrict__ rowidx, int m,
         const int e = ld_relaxed(&ctl->e);
         if (e >= m || ld_relaxed(&ctl->stalled)) break;
         const int len = e - s;
+        if (len <= kNarrow) {
+            // narrow frontier: workgroup 0 walks consecutive narrow levels
+            // with workgroup barriers only; the rest wait at the grid sync
+            if (blockIdx.x == 0)
+                bfs_narrow(colptr, rowidx, m, rem, par, levelItem, levelPtr,
+                           ctl, lds);
+            g.sync();
+            continue;
+        }
         // A: decrement dependants, raise their parent position
         for (int p = s + gtid; p < e; p += gsz) {
             const int u = levelItem[p];

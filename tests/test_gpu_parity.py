@@ -118,6 +118,38 @@ def test_solve_matches_oracle(oracle, torch, name, make, lower, lv, schedule):
     assert residual(f, x, b) < RES_TOL
 
 
+@pytest.mark.parametrize("lower", [True, False])
+def test_wide_and_narrow_levels(oracle, torch, lower):
+    """400k-row power-law factor: 47 of its 91 levels are wider than one
+    workgroup's frontier, so both paths of the level analysis run."""
+    f = gen.powerlaw(400_000, seed=3)
+    if not lower:
+        f = f.transpose()
+    _say("wide levels", "lower" if lower else "upper")
+    lv = 3
+    re, part, bnz, an = oracle.full_pipeline(f.colptr, f.rowidx, f.val, f.n,
+                                             f.substitution, lv)
+    p = _plan(f, lv)
+    li, lp = p.levels()
+    assert np.array_equal(lp, re["levelPtr"])
+    assert np.array_equal(li, re["levelItem"])
+    bl = p.blocks()
+    assert np.array_equal(bl["blk_nnz"], bnz)
+    for k in ("method", "nlv", "nnzr", "m_new", "lenmax", "longrow", "longlen"):
+        assert np.array_equal(bl[k], an[k]), k
+    for t in range(1 << lv):
+        lvp, lvi, nlv = p.tri_levels(t)
+        assert np.array_equal(lvp, an["lvptr_local"][t])
+        assert np.array_equal(lvi, an["lvitem_local"][t])
+    b = gen.rhs(f.n, seed=5)
+    xo = oracle.solve_blocks(re, f.n, lv, f.substitution, part, b)
+    for sched in (1, 2):
+        q = _plan(f, lv, schedule=sched)
+        x = q.solve(torch.as_tensor(b, device="cuda")).cpu().numpy()
+        q.check()
+        assert rel_err(x, xo) < X_TOL, sched
+
+
 def test_handworked_fixture_on_device(oracle, torch):
     import json
     import os
