@@ -50,7 +50,10 @@ typedef enum {
 typedef struct {
     int schedule;      /* RB_SCHED_* */
     int device_id;     /* device whose lv auto-threshold applies (lv = -1) */
-    int reserved[6];
+    int fused_waves;   /* fused schedule: cap on consumer waves (0 = auto) */
+    int no_graph;      /* blocked schedule: 1 = launch the block walk kernel
+                          by kernel instead of replaying a captured graph */
+    int reserved[4];
 } rb_opts;
 
 typedef struct rb_plan rb_plan;

@@ -164,12 +164,14 @@ rb_status rb_plan_create(const int *d_colptr, const int *d_rowidx,
     (void)hipGetDevice(&p->device);
     p->schedule = RB_SCHED_BLOCKED;
     p->want_schedule = o.schedule;
-    p->fused_waves = o.reserved[0];
+    p->fused_waves = o.fused_waves;
+    p->graph_failed = o.no_graph != 0;  // eager launches only
     rb_status st = build_plan(p, d_colptr, d_rowidx, d_val,
                               (hipStream_t)stream);
     if (st == RB_OK) st = choose_schedule(p, (hipStream_t)stream);
     if (st != RB_OK) {
         (void)hipStreamSynchronize((hipStream_t)stream);
+        destroy_graph(p);
         fused_destroy(p);
         p->arena.release();
         delete p;
@@ -216,6 +218,7 @@ rb_status rb_plan_check(rb_plan *p, void *stream) {
 rb_status rb_plan_destroy(rb_plan *p) {
     if (!p) return ok();
     (void)hipDeviceSynchronize();
+    destroy_graph(p);
     fused_destroy(p);
     p->arena.release();
     delete p;

@@ -1,31 +1,35 @@
 // Fused schedule: the whole solve in one persistent launch.
 //
-// Rows are grouped into "chains" that each belong to one consumer wave.  A
+// Rows are grouped into chains that each belong to one consumer wave.  A
 // wave solves its rows in level order, one *step* (at most 64 rows of one
-// level, one per lane) at a time.  A dependency on a row solved by the same
-// wave within the last R steps is read from the wave's LDS ring of recent x
-// values; every other dependency is a *global* one: it is read from the x
-// array in solve order, where each slot holds a sentinel NaN until solved.
+// level, one per lane) at a time.  Every off-diagonal entry of a row is read
+// from one of three places:
+//   previous step, lane l-1 / l / l+1   registers, moved across lanes by DPP
+//   same wave, within the last R steps  the wave's LDS ring of recent x
+//   anything else ("global")            x in slot order in device memory,
+//                                       a sentinel NaN until solved
 //
 // Assignment (host, at plan build): levels are walked in order; a row joins
 // the wave of its BFS parent (the dependency solved last) while that wave
 // has room at this level, otherwise a new wave (up to W) or the least
 // loaded one.  On grid-like matrices this keeps every row chain inside one
-// wave, so the per-level hop is an LDS round trip instead of a trip through
-// the memory hierarchy.
+// wave and most entries on the DPP path, so a level costs a few register
+// operations instead of a trip through the memory hierarchy.
 //
-// One workgroup per consumer wave, plus helper waves:
-//   feeders   copy the step records (row ids, 1/diag, LDS-class entries, b)
-//             from HBM into an LDS ring with asynchronous LDS-DMA loads
-//             (global_load_lds_dwordx4), several steps ahead;
+// One workgroup per consumer wave, plus a resolver wave:
+//   consumer  streams its step records (flags, 1/diag, coefficients, ring
+//             sources), the row ids and b straight into registers several
+//             steps ahead; per step: x = (b - fifo - ring - dpp) / diag,
+//             written to the LDS ring, to the slot-order x array when other
+//             waves read it, and to the caller's x;
 //   resolver  polls the global dependencies in order, many loads in flight,
-//             and hands the consumer one partial sum per (step, lane) through
-//             an LDS FIFO;
-//   consumer  x = (b - fifo - sum(val * xring[slot])) / diag, writes x to the
-//             LDS ring, to the solve-order x array (for other waves) and to
-//             the caller's x.
+//             and hands the consumer one partial sum per (step, lane)
+//             through an LDS FIFO.
+// The slot-order x array is double buffered: a solve reads and writes one
+// buffer and re-arms the published slots of the other with the sentinel,
+// so a solve is one launch with no preparation pass.
 // Every wait is bounded: on a timeout the kernel sets a status flag and
-// releases the waiting rows with NaN instead of hanging.
+// stops instead of hanging.
 #include <algorithm>
 #include <array>
 #include <chrono>
@@ -39,32 +43,34 @@ namespace rb {
 
 namespace {
 
-typedef __attribute__((address_space(3))) void lds_void;
-
-constexpr int kR = 32;          // LDS x ring depth (steps)
-constexpr int kFF = 512;        // FIFO capacity (groups)
-constexpr int kU = 4;           // resolver batches of 64 polled together
-constexpr int kH = 4;           // steps per feeder chunk
+constexpr int kR = 32;          // LDS x ring depth (steps), a power of two
+constexpr int kFF = 1024;       // FIFO capacity (groups), a power of two
+constexpr int kU = 2;           // resolver: batches of 64 polled together
+constexpr int kSigLag = 128;    // consumer reports FIFO use at this lag
 constexpr unsigned kSpin = 1u << 22;
-constexpr int kZeroSlot = kR * 64;
+constexpr int kZeroOff = kR * 64 * 8;  // byte offset of the ring's zero
 
-// s_waitcnt immediate that waits for vmcnt <= n only (gfx9 encoding)
-constexpr int vm_imm(int n) {
-    return (n & 0xF) | ((n >> 4) << 14) | (0x7 << 4) | (0xF << 8);
-}
+// record flags (int 0 of a lane's record)
+constexpr int kHasGlobal = 1;      // the row has a global group
+// bits 1..15: group offset (groups of lower lanes in this step)
+constexpr int kPublish = 1 << 30;  // other waves read this x
 
-// record layout: ops of 16 bytes per lane
-//   op0           {int q, int orig, double invd}
-//   op1..         {int hg, int s0, int s1, int s2} {s3..s6} {s7, ...}
-//   vals          {v0, v1} {v2, v3} ...
-//   b (own array) {double b, 0}
-template <int EC>
-struct Layout {
-    static constexpr int kSrcOps = EC <= 3 ? 1 : (EC <= 7 ? 2 : 3);
-    static constexpr int kValOps = (EC + 1) / 2;
-    static constexpr int kOps = 1 + kSrcOps + kValOps;  // without b
-    static constexpr int kP = EC <= 2 ? 16 : 8;          // ring steps
-    static constexpr int kNF = kP / kH;                  // feeder waves
+// Per-lane record of one step, as 32-bit words:
+//   [0] flags  [1..2] 1/diag
+//   NDPP: [3..4] coefficient of lane l (previous step), [5..6] of lane l-1,
+//         [7..8] of lane l+1
+//   NRING ring sources (byte offsets into the ring), NRING coefficients
+// padded to whole 16-byte words; word k of step s for lane l lives at
+// rec[(s * K + k) * 64 + l].
+template <int NDPP, int NRING>
+struct Fmt {
+    static constexpr int kSrc = 3 + (NDPP ? 6 : 0);
+    static constexpr int kVal = kSrc + NRING;
+    static constexpr int kInts = kVal + 2 * NRING;
+    static constexpr int K = (kInts + 3) / 4;
+    // prefetch distance in steps (registers: D * (4K + 2) + 2D)
+    static constexpr int D =
+        K <= 1 ? 16 : (K <= 3 ? 10 : (K <= 4 ? 8 : (K <= 6 ? 6 : (K <= 7 ? 5 : 4))));
 };
 
 __device__ inline int lds_load(const int *p) {
@@ -92,33 +98,34 @@ __device__ inline void st_x(double *p, double v) {
 __device__ inline bool pending(double v) {
     return __double_as_longlong(v) == (long long)kPending;
 }
+__device__ inline int status_now(const int *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline double dbl(int lo, int hi) { return __hiloint2double(hi, lo); }
 
 struct FusedArgs {
-    const int *step0;   // [W+1] first step of each wave
-    const int *gent0;   // [W+1] first global entry of each wave
-    const int4 *rec;    // [nsteps * ops * 64]
-    const double2 *brec;// [nsteps * 64]
-    const int *gsrc;    // [nent] solve-order index of the dependency
-    const double *gval; // [nent]
-    const int *gmeta;   // [nent] group ordinal * 2 + last-of-group
-    double *xq;         // [m] x in solve order (sentinel until solved)
-    double *x;          // caller's x (original order)
+    const int *step0;    // [W+1] first step of each wave
+    const int *gent0;    // [W+1] first global entry of each wave
+    const int *pub0;     // [W+1] first published slot of each wave
+    const int4 *rec;     // [nsteps * K * 64] step records
+    const int *orig;     // [nsteps * 64] original row of each slot, -1 none
+    const double *b;     // caller's b (original order)
+    const int *gsrc;     // [nent] slot of the dependency
+    const double *gval;  // [nent]
+    const int *gmeta;    // [nent] group ordinal * 2 + last-of-group
+    const int *pub;      // [npub] published slots, grouped by wave
+    double *xq;          // [nsteps * 64] x in slot order (this solve)
+    double *xq_next;     // the other buffer, re-armed for the next solve
+    double *x;           // caller's x (original order)
     int *status;
     // optional trace: per workgroup 16 counters of wall-clock ticks
-    // (10 ns): consumer wait-record, wait-fifo, total, steps; feeder
-    // wait-space, wait-dma, chunks; resolver wait-space, poll, batches,
-    // rounds
+    // (10 ns): [1] consumer FIFO wait, [2] consumer total, [3] steps,
+    // [7] resolver FIFO-space wait, [8] resolver poll, [9] batches,
+    // [10] poll rounds
     unsigned long long *trace;
 };
 
 __device__ inline unsigned long long now() { return wall_clock64(); }
-
-// source codes of a record entry (non-negative: LDS ring slot)
-constexpr int kPrevSame = -1;   // previous step, same lane (register)
-constexpr int kPrevLower = -2;  // previous step, lane - 1 (DPP)
-constexpr int kPrevUpper = -3;  // previous step, lane + 1 (DPP)
-constexpr int kPad = -4;        // padding, contributes nothing
-constexpr int kPublish = 1 << 30;  // hg flag: other waves read this x
 
 // lane l receives lane l-1's value (lane 0 receives 0)
 __device__ inline double dpp_from_lower(double v) {
@@ -133,170 +140,154 @@ __device__ inline double dpp_from_upper(double v) {
     return __hiloint2double(hi, lo);
 }
 
-// One lane's step record, read from an LDS ring slot (ops of 16 bytes).
-template <int EC>
-struct Rec {
-    int q, orig, hg;
-    double invd, b;
-    int src[EC];
-    double val[EC];
-    __device__ void load(const int4 (*slot)[64], int lane) {
-        using L = Layout<EC>;
-        const int4 o0 = slot[0][lane];
-        q = o0.x;
-        orig = o0.y;
-        invd = __hiloint2double(o0.w, o0.z);
-#pragma unroll
-        for (int k = 0; k < L::kSrcOps; k++) {
-            const int4 o = slot[1 + k][lane];
-            const int vv[4] = {o.x, o.y, o.z, o.w};
-#pragma unroll
-            for (int t = 0; t < 4; t++) {
-                const int idx = k * 4 + t - 1;
-                if (k == 0 && t == 0) hg = vv[0];
-                else if (idx < EC) src[idx] = vv[t];
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < L::kValOps; k++) {
-            const int4 o = slot[1 + L::kSrcOps + k][lane];
-            val[2 * k] = __hiloint2double(o.y, o.x);
-            if (2 * k + 1 < EC) val[2 * k + 1] = __hiloint2double(o.w, o.z);
-        }
-        const int4 ob = slot[L::kOps][lane];
-        b = __hiloint2double(ob.y, ob.x);
-    }
-};
-
 struct Shared {
-    int rec_ready[64];  // per feeder chunk: chunk index + 1 once loaded
-    int cons;           // steps consumed
-    int fifo_ready;     // groups resolved
-    int fifo_cons;      // groups consumed
+    int ready;  // groups resolved (resolver -> consumer)
+    int cons;   // groups consumed (consumer -> resolver)
     int abort;
 };
 
-template <int EC>
-__global__ void __launch_bounds__(64 * (2 + Layout<EC>::kNF))
-k_fused(FusedArgs a) {
-    using L = Layout<EC>;
-    constexpr int P = L::kP, NF = L::kNF, K = L::kOps;
-    __shared__ int4 ring[P][K + 1][64];   // +1: b
-    __shared__ double xring[kR * 64 + 1];
-    __shared__ double fifo[kFF];
-    __shared__ Shared sh;
-    __shared__ double rtmp[64];
-    __shared__ int gtmp[64];
-    __shared__ double ctmp;
+// Consumer wave.  The step body is instantiated once per position U of a
+// block of DO steps, so every pipeline register index is a compile-time
+// constant and the prefetch buffers stay in registers.
+template <int NDPP, int NRING>
+struct Consumer {
+    using F = Fmt<NDPP, NRING>;
+    static constexpr int K = F::K, D = F::D, DO = 2 * D;
 
-    const int w = blockIdx.x;
-    const int wave = threadIdx.x >> 6;
-    const int lane = threadIdx.x & 63;
-    const int s0 = a.step0[w];
-    const int ns = a.step0[w + 1] - s0;
+    const FusedArgs &a;
+    const int lane, ns, last;
+    const int4 *rec;
+    const int *orp;
+    double *xq;
+    double *xring;
+    const double *fifo;
+    Shared &sh;
+    // register pipeline: row ids DO steps ahead, records and b D steps ahead
+    int ob[DO];
+    int4 rb[D][K];
+    double bb[D];
+    int fifo_pos = 0, fifo_avail = 0, sig = 0;
+    double xprev = 0.0;
+    bool ok = true, go = true;
+    unsigned long long t_fifo = 0;
 
-    if (threadIdx.x == 0) {
-        for (int i = 0; i < 64; i++) sh.rec_ready[i] = 0;
-        sh.cons = 0;
-        sh.fifo_ready = 0;
-        sh.fifo_cons = 0;
-        sh.abort = 0;
-        xring[kZeroSlot] = 0.0;
+    __device__ __forceinline__ Consumer(const FusedArgs &a_, int w, int lane_,
+                                        double *xring_, const double *fifo_,
+                                        Shared &sh_)
+        : a(a_), lane(lane_), ns(a_.step0[w + 1] - a_.step0[w]),
+          last(a_.step0[w + 1] - a_.step0[w] - 1),
+          rec(a_.rec + (size_t)a_.step0[w] * K * 64 + lane_),
+          orp(a_.orig + (size_t)a_.step0[w] * 64 + lane_),
+          xq(a_.xq + (size_t)a_.step0[w] * 64 + lane_), xring(xring_),
+          fifo(fifo_), sh(sh_) {}
+
+    __device__ __forceinline__ void load_rec(int4 (&dst)[K], int t) {
+        const int4 *p = rec + (size_t)min(t, last) * K * 64;
+#pragma unroll
+        for (int k = 0; k < K; k++) dst[k] = p[k * 64];
     }
-    __syncthreads();
 
-    if (wave == 0) {
-        // ------------------------------------------------------- consumer
-        int fifo_pos = 0;
-        int rec_upto = 0;     // steps whose records are known to be in LDS
-        int fifo_avail = 0;   // groups known to be resolved
-        double xprev = 0.0;   // this lane's x of the previous step
-        unsigned long long t_rec = 0, t_fifo = 0, t_start = now();
-        // wait until the records of step s are in the ring (checks whole
-        // chunks, so most steps pay no LDS poll)
-        auto wait_rec = [&](int s) -> bool {
-            if (s < rec_upto) return true;
-            unsigned long long tw = a.trace ? now() : 0;
-            unsigned spins = 0;
-            while (true) {
-                const int c = rec_upto / kH;
-                if (lds_load(&sh.rec_ready[c % 64]) == c + 1) {
-                    rec_upto += kH;
-                    if (s < rec_upto) break;
-                    continue;
-                }
-                if (++spins > kSpin || lds_load(&sh.abort)) return false;
-                __builtin_amdgcn_s_sleep(1);
+    template <int U>
+    __device__ __forceinline__ void prime() {
+        if constexpr (U < DO) {
+            ob[U] = orp[(size_t)min(U, last) * 64];
+            prime<U + 1>();
+        }
+    }
+    template <int U>
+    __device__ __forceinline__ void prime_rec() {
+        if constexpr (U < D) {
+            load_rec(rb[U], U);
+            bb[U] = (U < ns && ob[U] >= 0) ? a.b[ob[U]] : 0.0;
+            prime_rec<U + 1>();
+        }
+    }
+
+    template <int U>
+    __device__ __forceinline__ void step(int sb) {
+        if constexpr (U < DO) {
+            constexpr int ur = U % D;
+            const int s = sb + U;
+            if (s >= ns) {
+                go = false;
+                return;
             }
-            wg_acquire();
-            if (a.trace) t_rec += now() - tw;
-            return true;
-        };
-        Rec<EC> cur, nxt;
-        bool ok = ns > 0 ? wait_rec(0) : true;
-        if (ns > 0 && ok) cur.load(ring[0], lane);
-        for (int s = 0; s < ns && ok; s++) {
-            // prefetch the next step's record while this one computes
-            const bool pre = s + 1 < ns && s + 1 < rec_upto;
-            if (pre) nxt.load(ring[(s + 1) % P], lane);
-            const int has = cur.hg & 1;
-            const int gofs = (cur.hg >> 1) & 0x7fff;
-            const int gtot = __shfl(gofs + has, 63, 64);
+            const int orig = ob[U];
+            int f[4 * K];
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                f[4 * k] = rb[ur][k].x;
+                f[4 * k + 1] = rb[ur][k].y;
+                f[4 * k + 2] = rb[ur][k].z;
+                f[4 * k + 3] = rb[ur][k].w;
+            }
+            const double bv = bb[ur];
+            // refill: row ids of step s + DO, record and b of step s + D
+            ob[U] = orp[(size_t)min(s + DO, last) * 64];
+            {
+                const int t = s + D;
+                const int o = ob[(U + D) % DO];
+                load_rec(rb[ur], t);
+                bb[ur] = (t < ns && o >= 0) ? a.b[o] : 0.0;
+            }
+
+            const int flags = f[0];
+            const double invd = dbl(f[1], f[2]);
             double acc = 0.0;
+            const int fl = __builtin_amdgcn_readlane(flags, 63);
+            const int gtot = ((fl >> 1) & 0x7fff) + (fl & kHasGlobal);
             if (gtot > 0) {
                 if (fifo_avail < fifo_pos + gtot) {
-                    unsigned long long tw = a.trace ? now() : 0;
+                    const unsigned long long tw = a.trace ? now() : 0;
                     unsigned spins = 0;
-                    while ((fifo_avail = lds_load(&sh.fifo_ready)) <
-                           fifo_pos + gtot) {
+                    while ((fifo_avail = lds_load(&sh.ready)) < fifo_pos + gtot) {
                         if (++spins > kSpin || lds_load(&sh.abort)) break;
                         __builtin_amdgcn_s_sleep(1);
                     }
-                    if (spins > kSpin || lds_load(&sh.abort)) {
-                        ok = false;
-                        break;
+                    if (fifo_avail < fifo_pos + gtot) {
+                        ok = go = false;
+                        return;
                     }
                     wg_acquire();
                     if (a.trace) t_fifo += now() - tw;
                 }
-                if (has) acc = fifo[(fifo_pos + gofs) % kFF];
+                const double fv =
+                    fifo[(fifo_pos + ((flags >> 1) & 0x7fff)) & (kFF - 1)];
+                acc = (flags & kHasGlobal) ? fv : 0.0;
+                fifo_pos += gtot;
             }
-            // previous step's x of this lane and of its neighbours
-            const double xl = dpp_from_lower(xprev), xr = dpp_from_upper(xprev);
 #pragma unroll
-            for (int j = 0; j < EC; j++) {
-                const int sj = cur.src[j];
-                double xv;
-                if (sj >= 0) xv = xring[sj];
-                else xv = sj == kPrevSame ? xprev
-                        : sj == kPrevLower ? xl
-                        : sj == kPrevUpper ? xr : 0.0;
-                acc += cur.val[j] * xv;
+            for (int j = 0; j < NRING; j++) {
+                const double xv =
+                    *(const double *)((const char *)xring + f[F::kSrc + j]);
+                acc += dbl(f[F::kVal + 2 * j], f[F::kVal + 2 * j + 1]) * xv;
             }
-            const double xv = (cur.b - acc) * cur.invd;
+            if constexpr (NDPP > 0) {
+                const double xl = dpp_from_lower(xprev);
+                const double xr = dpp_from_upper(xprev);
+                acc += dbl(f[3], f[4]) * xprev;
+                acc += dbl(f[5], f[6]) * xl;
+                acc += dbl(f[7], f[8]) * xr;
+            }
+            const double xv = (bv - acc) * invd;
             xprev = xv;
-            xring[(s % kR) * 64 + lane] = xv;
-            if (cur.q >= 0) {
-                if (cur.hg & kPublish) st_x(&a.xq[cur.q], xv);
-                a.x[cur.orig] = xv;
+            if constexpr (NRING > 0) xring[(s & (kR - 1)) * 64 + lane] = xv;
+            if (orig >= 0) a.x[orig] = xv;
+            if (flags & kPublish) st_x(&xq[(size_t)s * 64], xv);
+            if (fifo_pos - sig >= kSigLag || (U == DO - 1 && fifo_pos != sig)) {
+                wg_release();
+                if (lane == 0) lds_store(&sh.cons, fifo_pos);
+                sig = fifo_pos;
             }
-            fifo_pos += gtot;
-            wg_release();
-            if (lane == 0) {
-                lds_store(&sh.fifo_cons, fifo_pos);
-                lds_store(&sh.cons, s + 1);
-            }
-            if (s + 1 < ns) {
-                if (!pre) {
-                    if (!wait_rec(s + 1)) {
-                        ok = false;
-                        break;
-                    }
-                    nxt.load(ring[(s + 1) % P], lane);
-                }
-                cur = nxt;
-            }
+            step<U + 1>(sb);
         }
+    }
+
+    __device__ __forceinline__ void run(int w) {
+        const unsigned long long t0 = a.trace ? now() : 0;
+        prime<0>();
+        prime_rec<0>();
+        for (int sb = 0; go; sb += DO) step<0>(sb);
         if (!ok && lane == 0) {
             atomicOr(a.status, 2);
             lds_store(&sh.abort, 1);
@@ -304,103 +295,70 @@ k_fused(FusedArgs a) {
         if (lane == 0) lds_store(&sh.cons, 1 << 30);
         if (a.trace && lane == 0) {
             unsigned long long *T = a.trace + (size_t)w * 16;
-            T[0] = t_rec;
             T[1] = t_fifo;
-            T[2] = now() - t_start;
+            T[2] = now() - t0;
             T[3] = ns;
         }
-    } else if (wave <= NF) {
-        // ------------------------------------------------------- feeders
-        const int f = wave - 1;
-        const int4 *rec = a.rec + (size_t)s0 * K * 64;
-        const double2 *brec = a.brec + (size_t)s0 * 64;
-        const int nchunks = (ns + kH - 1) / kH;
-        unsigned long long t_space = 0, t_dma = 0, tw = 0;
-        for (int c = f; c < nchunks; c += NF) {
-            const int first = c * kH;
-            if (a.trace) tw = now();
-            // the ring holds P steps: wait until steps < first + kH - P are
-            // consumed
+    }
+};
+
+__device__ __forceinline__ void resolve(const FusedArgs &a, int w, int lane,
+                                        double *fifo, double *rtmp, int *gtmp,
+                                        double *ctmp, Shared &sh) {
+    // re-arm this wave's published slots of the next solve's buffer; no
+    // wave of this launch reads that buffer
+    for (int i = a.pub0[w] + lane; i < a.pub0[w + 1]; i += 64)
+        ((unsigned long long *)a.xq_next)[a.pub[i]] = kPending;
+
+    const int e0 = a.gent0[w], e1 = a.gent0[w + 1];
+    double carry = 0.0;
+    int carry_g = -1;
+    int done_groups = 0;
+    bool abort = false;
+    unsigned long long t_rspace = 0, t_poll = 0, nb = 0, nround = 0, tw = 0;
+    for (int kb = e0; kb < e1 && !abort; kb += 64 * kU) {
+        // metadata and first polls of kU batches of 64 entries at once, so
+        // kU * 64 loads are in flight per round trip
+        if (a.trace) tw = now();
+        int srcu[kU], gmu[kU];
+        double gvu[kU], xvu[kU];
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const int idx = kb + 64 * u + lane;
+            srcu[u] = 0;
+            gmu[u] = 0;
+            gvu[u] = 0.0;
+            if (idx < e1) {
+                srcu[u] = a.gsrc[idx];
+                gvu[u] = a.gval[idx];
+                gmu[u] = a.gmeta[idx];
+            }
+        }
+        {
+            // FIFO space for every group these batches may complete
+            const int nt = min(64 * kU, e1 - kb);
+            const int gmax = a.gmeta[kb + nt - 1] >> 1;
             unsigned spins = 0;
-            while (lds_load(&sh.cons) < first + kH - P) {
-                if (++spins > kSpin || lds_load(&sh.abort)) break;
+            while (gmax - lds_load(&sh.cons) >= kFF) {
+                if (++spins > kSpin || lds_load(&sh.abort)) {
+                    abort = true;
+                    break;
+                }
                 __builtin_amdgcn_s_sleep(1);
             }
-            if (spins > kSpin || lds_load(&sh.abort)) break;
-            if (a.trace) {
-                t_space += now() - tw;
-                tw = now();
-            }
-            for (int t = 0; t < kH; t++) {
-                const int s = first + t;
-                if (s >= ns) break;
-                const int slot = s % P;
-#pragma unroll
-                for (int k = 0; k < K; k++)
-                    __builtin_amdgcn_global_load_lds(
-                        (const void *)(rec + ((size_t)s * K + k) * 64 + lane),
-                        (lds_void *)&ring[slot][k][0], 16, 0, 0);
-                __builtin_amdgcn_global_load_lds(
-                    (const void *)(brec + (size_t)s * 64 + lane),
-                    (lds_void *)&ring[slot][K][0], 16, 0, 0);
-            }
-            __builtin_amdgcn_s_waitcnt(vm_imm(0));
-            if (a.trace) t_dma += now() - tw;
-            wg_release();
-            if (lane == 0) lds_store(&sh.rec_ready[c % 64], c + 1);
+            if (abort) break;
         }
-        if (a.trace && lane == 0 && f == 0) {
-            unsigned long long *T = a.trace + (size_t)w * 16;
-            T[4] = t_space;
-            T[5] = t_dma;
-            T[6] = nchunks;
+        if (a.trace) {
+            t_rspace += now() - tw;
+            tw = now();
         }
-    } else {
-        // ------------------------------------------------------- resolver
-        const int e0 = a.gent0[w], e1 = a.gent0[w + 1];
-        double carry = 0.0;
-        int carry_g = -1;
-        int done_groups = 0;
-        unsigned long long t_rspace = 0, t_poll = 0, nb = 0, nround = 0, tw = 0;
-        for (int kb = e0; kb < e1; kb += 64 * kU) {
-            // metadata and first polls of kU batches of 64 entries at once,
-            // so kU * 64 loads are in flight per round trip
-            if (a.trace) tw = now();
-            int srcu[kU], gmu[kU];
-            double gvu[kU], xvu[kU];
 #pragma unroll
-            for (int u = 0; u < kU; u++) {
-                const int idx = kb + 64 * u + lane;
-                srcu[u] = 0;
-                gmu[u] = 0;
-                gvu[u] = 0.0;
-                if (idx < e1) {
-                    srcu[u] = a.gsrc[idx];
-                    gvu[u] = a.gval[idx];
-                    gmu[u] = a.gmeta[idx];
-                }
-            }
-            {
-                // FIFO space for every group these batches may complete
-                const int nt = min(64 * kU, e1 - kb);
-                const int gmax = a.gmeta[kb + nt - 1] >> 1;
-                unsigned spins = 0;
-                while (gmax - lds_load(&sh.fifo_cons) >= kFF) {
-                    if (++spins > kSpin || lds_load(&sh.abort)) break;
-                    __builtin_amdgcn_s_sleep(1);
-                }
-            }
-            if (a.trace) {
-                t_rspace += now() - tw;
-                tw = now();
-            }
+        for (int u = 0; u < kU; u++)
+            xvu[u] = kb + 64 * u + lane < e1 ? ld_x(&a.xq[srcu[u]]) : 0.0;
 #pragma unroll
-            for (int u = 0; u < kU; u++)
-                xvu[u] = kb + 64 * u + lane < e1 ? ld_x(&a.xq[srcu[u]]) : 0.0;
-#pragma unroll
-            for (int u = 0; u < kU; u++) {
+        for (int u = 0; u < kU; u++) {
             const int k0 = kb + 64 * u;
-            if (k0 >= e1) break;
+            if (k0 >= e1 || abort) break;
             nb++;
             const int n = min(64, e1 - k0);
             const bool act = lane < n;
@@ -442,28 +400,31 @@ k_fused(FusedArgs a) {
                             if (l + 1 >= hi || (gtmp[l + 1] >> 1) != g) break;
                             l++;
                         }
-                        if (gtmp[l] & 1) fifo[g % kFF] = sum;  // complete
-                        else ctmp = sum;  // only the batch's last group
+                        if (gtmp[l] & 1) fifo[g & (kFF - 1)] = sum;  // complete
+                        else *ctmp = sum;  // only the batch's last group
                     }
                     const int gl = __shfl(gm, hi - 1, 64);
                     if (gl & 1) {
                         done_groups = (gl >> 1) + 1;
                     } else {  // hi == n: the last group continues
-                        carry = ctmp;
+                        carry = *ctmp;
                         done_groups = gl >> 1;
                     }
                     carry_g = (gl & 1) ? -1 : (gl >> 1);
                     wg_release();
-                    if (lane == 0) lds_store(&sh.fifo_ready, done_groups);
+                    if (lane == 0) lds_store(&sh.ready, done_groups);
                     done_lo = hi;
                 }
                 if (fp >= n) break;
                 nround++;
-                if (++spins > kSpin || lds_load(&sh.abort)) {
+                ++spins;
+                if (spins > kSpin || lds_load(&sh.abort) ||
+                    ((spins & 1023) == 0 && status_now(a.status))) {
                     if (lane == 0) {
                         atomicOr(a.status, 4);
                         lds_store(&sh.abort, 1);
                     }
+                    abort = true;
                     break;
                 }
                 __builtin_amdgcn_s_sleep(1);
@@ -473,30 +434,48 @@ k_fused(FusedArgs a) {
                     if (!pend) rtmp[lane] = gv * xv;
                 }
             }
-            if (lds_load(&sh.abort)) break;
-            }
-            if (a.trace) t_poll += now() - tw;
-            if (lds_load(&sh.abort)) break;
         }
-        if (a.trace && lane == 0) {
-            unsigned long long *T = a.trace + (size_t)w * 16;
-            T[7] = t_rspace;
-            T[8] = t_poll;
-            T[9] = nb;
-            T[10] = nround;
-        }
+        if (a.trace) t_poll += now() - tw;
+        if (lds_load(&sh.abort)) abort = true;
+    }
+    if (a.trace && lane == 0) {
+        unsigned long long *T = a.trace + (size_t)w * 16;
+        T[7] = t_rspace;
+        T[8] = t_poll;
+        T[9] = nb;
+        T[10] = nround;
     }
 }
 
-__global__ void k_fused_prologue(const int *__restrict__ orig, long nslots,
-                                 const double *__restrict__ b,
-                                 double2 *brec, double *xq, int m) {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nslots) {
-        const int o = orig[i];
-        brec[i] = make_double2(o >= 0 ? b[o] : 0.0, 0.0);
+template <int NDPP, int NRING>
+__global__ void __launch_bounds__(128) k_fused(FusedArgs a) {
+    __shared__ double xring[NRING > 0 ? kR * 64 + 1 : 1];
+    __shared__ double fifo[kFF];
+    __shared__ double rtmp[64];
+    __shared__ int gtmp[64];
+    __shared__ double ctmp;
+    __shared__ Shared sh;
+
+    const int w = blockIdx.x;
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    if (threadIdx.x == 0) {
+        sh.ready = 0;
+        sh.cons = 0;
+        sh.abort = 0;
+        if (NRING > 0) xring[kR * 64] = 0.0;
     }
-    if (i < m) ((unsigned long long *)xq)[i] = kPending;
+    __syncthreads();
+    if (wave == 0) {
+        Consumer<NDPP, NRING> c(a, w, lane, xring, fifo, sh);
+        c.run(w);
+    } else
+        resolve(a, w, lane, fifo, rtmp, gtmp, &ctmp, sh);
+}
+
+__global__ void k_fill_pending(double *p, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) ((unsigned long long *)p)[i] = kPending;
 }
 
 }  // namespace
@@ -504,25 +483,23 @@ __global__ void k_fused_prologue(const int *__restrict__ orig, long nslots,
 // ----------------------------------------------------------------- host
 
 struct FusedPlan {
-    int W = 0, EC = 0, nsteps = 0, nent = 0;
-    int *d_step0 = nullptr, *d_gent0 = nullptr, *d_orig = nullptr;
+    int W = 0, NDPP = 0, NRING = 0, K = 0, nsteps = 0, nent = 0, npub = 0;
+    unsigned epoch = 0;  // solves run; selects the slot-order x buffer
+    int *d_step0 = nullptr, *d_gent0 = nullptr, *d_pub0 = nullptr;
     int4 *d_rec = nullptr;
-    double2 *d_brec = nullptr;
-    int *d_gsrc = nullptr, *d_gmeta = nullptr;
+    int *d_orig = nullptr;
+    int *d_gsrc = nullptr, *d_gmeta = nullptr, *d_pub = nullptr;
     double *d_gval = nullptr;
-    double *d_xq = nullptr;
+    double *d_xq = nullptr;  // two buffers of nsteps * 64
     unsigned long long *d_trace = nullptr;  // RB_FUSED_TRACE=1
     double build_ms = 0.0;
 };
 
-static int ec_class(int e) { return e <= 2 ? 2 : (e <= 4 ? 4 : 8); }
+static int ring_class(int e) { return e == 0 ? 0 : (e <= 2 ? 2 : (e <= 4 ? 4 : 8)); }
 
-static int ops_for(int ec) {
-    switch (ec) {
-    case 2: return Layout<2>::kOps;
-    case 4: return Layout<4>::kOps;
-    default: return Layout<8>::kOps;
-    }
+static int words_for(int ndpp, int nring) {
+    const int ints = 3 + (ndpp ? 6 : 0) + 3 * nring;
+    return (ints + 3) / 4;
 }
 
 // Host partitioner and record builder.  rp/cc/cv: permuted CSR (ascending
@@ -562,7 +539,6 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
     int W = 0;
     std::vector<int> cnt;           // per wave count at current level
     std::vector<int> stamp;
-    std::vector<int> nsteps_w;      // steps used so far per wave
     std::vector<std::vector<int>> wave_rows;  // rows (q) per wave, in order
     int rr = 0;
     for (int L = 0; L < nlev; L++) {
@@ -581,7 +557,6 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
                     w = W++;
                     cnt.push_back(0);
                     stamp.push_back(-1);
-                    nsteps_w.push_back(0);
                     wave_rows.emplace_back();
                 } else {
                     for (int t = 0; t < W; t++) {
@@ -647,32 +622,58 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
             i = j;
         }
     }
-    // ---- classify entries, choose EC
+    // ---- entry classes: DPP (previous step, lane l-1 / l / l+1), ring
+    // (same wave, within kR steps), global (everything else)
+    auto is_dpp = [&](int q, int u) {
+        return wave[u] == wave[q] && stepw[q] - stepw[u] == 1 &&
+               std::abs(lanew[u] - lanew[q]) <= 1;
+    };
+    auto is_loc = [&](int q, int u) {
+        return wave[u] == wave[q] && stepw[q] - stepw[u] <= kR;
+    };
+    long n_all = 0, n_dpp = 0;
+    for (int q = 0; q < m; q++) {
+        const int r = pof(q);
+        for (int e = rp[r]; e < rp[r + 1]; e++) {
+            if (cc[e] == r) continue;
+            n_all++;
+            n_dpp += is_dpp(q, qof(cc[e]));
+        }
+    }
+    const int NDPP = (n_dpp > 0 && 4 * n_dpp >= n_all) ? 3 : 0;
     int emax = 0;
-    std::vector<int> nloc(m, 0);
     for (int q = 0; q < m; q++) {
         const int r = pof(q);
         int k = 0;
         for (int e = rp[r]; e < rp[r + 1]; e++) {
             if (cc[e] == r) continue;
             const int u = qof(cc[e]);
-            if (wave[u] == wave[q] && stepw[q] - stepw[u] <= kR) k++;
+            if (is_loc(q, u) && !(NDPP && is_dpp(q, u))) k++;
         }
-        nloc[q] = k;
         emax = std::max(emax, k);
     }
-    const int EC = ec_class(std::min(emax, 8));
-    const int K = ops_for(EC);
-    const int KS = EC <= 3 ? 1 : (EC <= 7 ? 2 : 3);
+    const int NRING = ring_class(std::min(emax, 8));
+    const int K = words_for(NDPP, NRING);
+    const int kSrc = 3 + (NDPP ? 6 : 0), kVal = kSrc + NRING;
     // ---- emit records
-    std::vector<int> step0(W + 1, 0), gent0(W + 1, 0);
+    std::vector<int> step0(W + 1, 0), gent0(W + 1, 0), pub0(W + 1, 0);
     for (int w = 0; w < W; w++) step0[w + 1] = step0[w] + (int)steps[w].size();
     const int NS = step0[W];
+    auto slot_of = [&](int q) {
+        return (step0[wave[q]] + stepw[q]) * 64 + lanew[q];
+    };
     std::vector<int4> rec((size_t)NS * K * 64);
     std::vector<int> origv((size_t)NS * 64, -1);
     std::vector<int> gsrc, gmeta;
     std::vector<double> gval;
     std::vector<char> needpub(m, 0);
+    std::vector<int> f(4 * K);
+    auto put = [&](int at, double v) {
+        long long bits;
+        std::memcpy(&bits, &v, 8);
+        f[at] = (int)(bits & 0xffffffff);
+        f[at + 1] = (int)(bits >> 32);
+    };
     for (int w = 0; w < W; w++) {
         gent0[w] = (int)gsrc.size();
         int gord = 0;
@@ -682,36 +683,27 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
             int gofs = 0;
             for (int l = 0; l < 64; l++) {
                 const int q = st[l];
-                int4 *op = &rec[sbase];
-                // defaults: inactive lane / padding
-                int srcs[8];
-                double vals[8];
-                for (int j = 0; j < 8; j++) {
-                    srcs[j] = kPad;
-                    vals[j] = 0.0;
-                }
-                int hg = gofs << 1;
+                std::fill(f.begin(), f.end(), 0);
+                for (int j = 0; j < NRING; j++) f[kSrc + j] = kZeroOff;
+                int flags = gofs << 1;
                 if (q >= 0) {
                     const int r = pof(q);
-                    const int orig = lf[r];
-                    origv[(size_t)(step0[w] + si) * 64 + l] = orig;
+                    origv[(size_t)(step0[w] + si) * 64 + l] = lf[r];
                     int nl = 0;
                     bool hasg = false;
                     for (int e = rp[r]; e < rp[r + 1]; e++) {
                         if (cc[e] == r) continue;
                         const int u = qof(cc[e]);
-                        const bool loc = wave[u] == w && si - stepw[u] <= kR;
-                        if (loc && nl < EC) {
-                            const int du = lanew[u] - l;
-                            if (stepw[u] == si - 1 && du == 0) srcs[nl] = kPrevSame;
-                            else if (stepw[u] == si - 1 && du == -1) srcs[nl] = kPrevLower;
-                            else if (stepw[u] == si - 1 && du == 1) srcs[nl] = kPrevUpper;
-                            else srcs[nl] = (stepw[u] % kR) * 64 + lanew[u];
-                            vals[nl] = cv[e];
+                        if (NDPP && is_dpp(q, u)) {
+                            const int du = lanew[u] - l;  // -1, 0, +1
+                            put(du == 0 ? 3 : (du < 0 ? 5 : 7), cv[e]);
+                        } else if (is_loc(q, u) && nl < NRING) {
+                            f[kSrc + nl] = ((stepw[u] % kR) * 64 + lanew[u]) * 8;
+                            put(kVal + 2 * nl, cv[e]);
                             nl++;
                         } else {
                             needpub[u] = 1;
-                            gsrc.push_back(u);
+                            gsrc.push_back(slot_of(u));
                             gval.push_back(cv[e]);
                             gmeta.push_back(gord * 2);
                             hasg = true;
@@ -720,62 +712,60 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
                     if (hasg) {
                         gmeta.back() |= 1;
                         gord++;
-                        hg = (gofs << 1) | 1;
+                        flags = (gofs << 1) | kHasGlobal;
                         gofs++;
                     }
-                    const double d = invd[r];
-                    long long dbits;
-                    std::memcpy(&dbits, &d, 8);
-                    op[0 * 64 + l] = make_int4(q, orig, (int)(dbits & 0xffffffff),
-                                               (int)(dbits >> 32));
-                } else {
-                    op[0 * 64 + l] = make_int4(-1, 0, 0, 0);
+                    put(1, invd[r]);
                 }
-                // sources
-                int flat[12];
-                flat[0] = hg;
-                for (int j = 0; j < 11; j++) flat[1 + j] = j < EC ? srcs[j] : kPad;
-                for (int k = 0; k < KS; k++)
-                    op[(1 + k) * 64 + l] = make_int4(flat[4 * k], flat[4 * k + 1],
-                                                     flat[4 * k + 2], flat[4 * k + 3]);
-                for (int k = 0; k < (EC + 1) / 2; k++) {
-                    long long v0, v1;
-                    std::memcpy(&v0, &vals[2 * k], 8);
-                    std::memcpy(&v1, &vals[2 * k + 1], 8);
-                    op[(1 + KS + k) * 64 + l] =
-                        make_int4((int)(v0 & 0xffffffff), (int)(v0 >> 32),
-                                  (int)(v1 & 0xffffffff), (int)(v1 >> 32));
-                }
+                f[0] = flags;
+                for (int k = 0; k < K; k++)
+                    rec[sbase + (size_t)k * 64 + l] =
+                        make_int4(f[4 * k], f[4 * k + 1], f[4 * k + 2], f[4 * k + 3]);
             }
         }
     }
-    // rows read by other waves (or beyond the LDS window) publish their x
-    for (int q = 0; q < m; q++)
-        if (needpub[q])
-            rec[((size_t)(step0[wave[q]] + stepw[q]) * K + 1) * 64 + lanew[q]].x |=
-                kPublish;
     gent0[W] = (int)gsrc.size();
+    // rows read by other waves (or beyond the ring window) publish their x
+    std::vector<std::vector<int>> pubw(W);
+    for (int q = 0; q < m; q++)
+        if (needpub[q]) {
+            const int sl = slot_of(q);
+            rec[(size_t)(sl / 64) * K * 64 + (sl % 64)].x |= kPublish;
+            pubw[wave[q]].push_back(sl);
+        }
+    std::vector<int> pub;
+    for (int w = 0; w < W; w++) {
+        pub0[w] = (int)pub.size();
+        pub.insert(pub.end(), pubw[w].begin(), pubw[w].end());
+    }
+    pub0[W] = (int)pub.size();
     if (gsrc.empty()) {
         gsrc.push_back(0);
         gval.push_back(0.0);
         gmeta.push_back(1);
     }
+    if (pub.empty()) pub.push_back(0);
     // ---- upload
     FusedPlan *F = new FusedPlan();
     F->W = W;
-    F->EC = EC;
+    F->NDPP = NDPP;
+    F->NRING = NRING;
+    F->K = K;
     F->nsteps = NS;
     F->nent = gent0[W];
+    F->npub = pub0[W];
     Arena &A = p->arena;
+    const size_t nslots = (size_t)NS * 64;
     RB_TRY(A.alloc(&F->d_step0, W + 1));
     RB_TRY(A.alloc(&F->d_gent0, W + 1));
+    RB_TRY(A.alloc(&F->d_pub0, W + 1));
     RB_TRY(A.alloc(&F->d_rec, rec.size()));
-    RB_TRY(A.alloc(&F->d_brec, (size_t)NS * 64));
-    RB_TRY(A.alloc(&F->d_orig, (size_t)NS * 64));
+    RB_TRY(A.alloc(&F->d_orig, nslots));
     RB_TRY(A.alloc(&F->d_gsrc, gsrc.size()));
     RB_TRY(A.alloc(&F->d_gval, gval.size()));
     RB_TRY(A.alloc(&F->d_gmeta, gmeta.size()));
-    RB_TRY(A.alloc(&F->d_xq, m));
+    RB_TRY(A.alloc(&F->d_pub, pub.size()));
+    RB_TRY(A.alloc(&F->d_xq, 2 * nslots));
     if (getenv("RB_FUSED_TRACE")) {
         RB_TRY(A.alloc(&F->d_trace, (size_t)W * 16));
         RB_HIP(hipMemsetAsync(F->d_trace, 0, sizeof(unsigned long long) * W * 16, s));
@@ -786,11 +776,18 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
     };
     RB_TRY(up(F->d_step0, step0.data(), sizeof(int) * step0.size()));
     RB_TRY(up(F->d_gent0, gent0.data(), sizeof(int) * gent0.size()));
+    RB_TRY(up(F->d_pub0, pub0.data(), sizeof(int) * pub0.size()));
     RB_TRY(up(F->d_rec, rec.data(), sizeof(int4) * rec.size()));
     RB_TRY(up(F->d_orig, origv.data(), sizeof(int) * origv.size()));
     RB_TRY(up(F->d_gsrc, gsrc.data(), sizeof(int) * gsrc.size()));
     RB_TRY(up(F->d_gval, gval.data(), sizeof(double) * gval.size()));
     RB_TRY(up(F->d_gmeta, gmeta.data(), sizeof(int) * gmeta.size()));
+    RB_TRY(up(F->d_pub, pub.data(), sizeof(int) * pub.size()));
+    {
+        const long n = 2 * (long)nslots;
+        k_fill_pending<<<(int)((n + 255) / 256), 256, 0, s>>>(F->d_xq, n);
+        RB_HIP(hipGetLastError());
+    }
     RB_HIP(hipStreamSynchronize(s));
     auto t1 = std::chrono::steady_clock::now();
     F->build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
@@ -837,16 +834,24 @@ int fused_max_waves() {
     return ncu > 0 ? ncu : 256;
 }
 
-template <int EC>
+template <int NDPP, int NRING>
 static rb_status launch_fused(FusedPlan *F, const FusedArgs &args,
                               hipStream_t s) {
     void *kargs[] = {(void *)&args};
-    const int threads = 64 * (2 + Layout<EC>::kNF);
-    hipError_t e = hipLaunchCooperativeKernel((const void *)k_fused<EC>,
-                                              dim3(F->W), dim3(threads), kargs,
-                                              0, s);
+    hipError_t e = hipLaunchCooperativeKernel(
+        (const void *)k_fused<NDPP, NRING>, dim3(F->W), dim3(128), kargs, 0, s);
     if (e != hipSuccess) return status_of_hip(e, "hipLaunchCooperativeKernel(k_fused)");
     return RB_OK;
+}
+
+template <int NDPP>
+static rb_status launch_ring(FusedPlan *F, const FusedArgs &args, hipStream_t s) {
+    switch (F->NRING) {
+    case 0: return launch_fused<NDPP, 0>(F, args, s);
+    case 2: return launch_fused<NDPP, 2>(F, args, s);
+    case 4: return launch_fused<NDPP, 4>(F, args, s);
+    default: return launch_fused<NDPP, 8>(F, args, s);
+    }
 }
 
 rb_status solve_fused(rb_plan *p, const double *d_b, int ldb, double *d_x,
@@ -856,24 +861,17 @@ rb_status solve_fused(rb_plan *p, const double *d_b, int ldb, double *d_x,
         set_error(RB_ERR_INTERNAL, "fused schedule not built");
         return RB_ERR_INTERNAL;
     }
-    const int m = p->m;
-    if (m == 0) return RB_OK;
-    const long nslots = (long)F->nsteps * 64;
+    if (p->m == 0) return RB_OK;
+    const size_t nslots = (size_t)F->nsteps * 64;
     for (int k = 0; k < nrhs; k++) {
-        const double *b = d_b + (size_t)k * ldb;
-        double *x = d_x + (size_t)k * ldx;
-        const long nt = std::max(nslots, (long)m);
-        k_fused_prologue<<<(int)((nt + 255) / 256), 256, 0, s>>>(
-            F->d_orig, nslots, b, F->d_brec, F->d_xq, m);
-        RB_HIP(hipGetLastError());
-        FusedArgs args{F->d_step0, F->d_gent0, F->d_rec, F->d_brec,
-                       F->d_gsrc, F->d_gval, F->d_gmeta, F->d_xq, x,
-                       p->d_status, F->d_trace};
-        switch (F->EC) {
-        case 2: RB_TRY(launch_fused<2>(F, args, s)); break;
-        case 4: RB_TRY(launch_fused<4>(F, args, s)); break;
-        default: RB_TRY(launch_fused<8>(F, args, s)); break;
-        }
+        double *cur = F->d_xq + (F->epoch & 1) * nslots;
+        double *nxt = F->d_xq + ((F->epoch + 1) & 1) * nslots;
+        FusedArgs args{F->d_step0, F->d_gent0, F->d_pub0, F->d_rec,
+                       F->d_orig, d_b + (size_t)k * ldb, F->d_gsrc,
+                       F->d_gval, F->d_gmeta, F->d_pub, cur, nxt,
+                       d_x + (size_t)k * ldx, p->d_status, F->d_trace};
+        RB_TRY(F->NDPP ? launch_ring<3>(F, args, s) : launch_ring<0>(F, args, s));
+        F->epoch++;
     }
     return RB_OK;
 }

@@ -142,6 +142,9 @@ struct rb_plan {
     int want_schedule = RB_SCHED_AUTO;
     int fused_waves = 0;        // wave cap for the fused partition (0 = auto)
     double tune_ms_blocked = 0.0, tune_ms_fused = 0.0;
+    // blocked schedule: the block walk captured once as a graph
+    void *graph_exec = nullptr;
+    bool graph_failed = false;
 };
 
 namespace rb {
@@ -155,5 +158,6 @@ rb_status build_fused(rb_plan *p, const int *d_prptr, const int *d_ccol,
 rb_status solve_fused(rb_plan *p, const double *d_b, int ldb, double *d_x,
                       int ldx, int nrhs, hipStream_t s);
 void fused_destroy(rb_plan *p);
+void destroy_graph(rb_plan *p);
 int fused_max_waves();
 }  // namespace rb

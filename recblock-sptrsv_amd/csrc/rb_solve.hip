@@ -251,16 +251,9 @@ inline int nblk(long n, int t = kT) { return (int)((n + t - 1) / t); }
 
 }  // namespace
 
-rb_status solve_blocked(rb_plan *p, const double *d_b, int ldb, double *d_x,
-                        int ldx, int nrhs, hipStream_t s) {
-    const int m = p->m;
-    if (m == 0) return RB_OK;
-    for (int k = 0; k < nrhs; k++) {
-        const double *b = d_b + (size_t)k * ldb;
-        double *x = d_x + (size_t)k * ldx;
-        const int pro = nblk(m > p->ntri ? m : p->ntri);
-        k_prologue<<<pro, kT, 0, s>>>(b, p->d_levelItem, m, p->d_bperm,
-                                      p->d_xperm, p->d_tickets, p->ntri);
+// The block walk of one solve: kernels on plan-owned buffers only, so it
+// can be captured once into a graph and replayed.
+static void launch_blocks(rb_plan *p, hipStream_t s) {
         for (int i = 0; i < p->nblocks; i++) {
             const Block &B = p->blocks[i];
             if (B.blk_m == 0) continue;
@@ -328,6 +321,54 @@ rb_status solve_blocked(rb_plan *p, const double *d_b, int ldb, double *d_x,
                         B.left, p->d_bperm);
             }
         }
+}
+
+// Capture the block walk into a graph (once per plan).  Returns false when
+// capture is not possible; the walk is then launched kernel by kernel.
+static bool capture_blocks(rb_plan *p) {
+    if (p->graph_failed) return false;
+    if (p->graph_exec) return true;
+    hipStream_t cs = nullptr;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    bool ok = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) == hipSuccess;
+    if (ok) ok = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    if (ok) {
+        launch_blocks(p, cs);
+        ok = hipStreamEndCapture(cs, &g) == hipSuccess && g != nullptr;
+    }
+    if (ok) ok = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) == hipSuccess;
+    if (g) (void)hipGraphDestroy(g);
+    if (cs) (void)hipStreamDestroy(cs);
+    (void)hipGetLastError();
+    if (!ok) {
+        p->graph_failed = true;
+        return false;
+    }
+    p->graph_exec = ge;
+    return true;
+}
+
+void destroy_graph(rb_plan *p) {
+    if (p->graph_exec) (void)hipGraphExecDestroy((hipGraphExec_t)p->graph_exec);
+    p->graph_exec = nullptr;
+}
+
+rb_status solve_blocked(rb_plan *p, const double *d_b, int ldb, double *d_x,
+                        int ldx, int nrhs, hipStream_t s) {
+    const int m = p->m;
+    if (m == 0) return RB_OK;
+    const bool graph = capture_blocks(p);
+    for (int k = 0; k < nrhs; k++) {
+        const double *b = d_b + (size_t)k * ldb;
+        double *x = d_x + (size_t)k * ldx;
+        const int pro = nblk(m > p->ntri ? m : p->ntri);
+        k_prologue<<<pro, kT, 0, s>>>(b, p->d_levelItem, m, p->d_bperm,
+                                      p->d_xperm, p->d_tickets, p->ntri);
+        if (graph)
+            RB_HIP(hipGraphLaunch((hipGraphExec_t)p->graph_exec, s));
+        else
+            launch_blocks(p, s);
         k_epilogue<<<nblk(m), kT, 0, s>>>(p->d_xperm, p->d_levelItem, m, x);
         RB_HIP(hipGetLastError());
     }

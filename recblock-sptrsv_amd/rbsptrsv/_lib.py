@@ -39,7 +39,8 @@ class RbError(RuntimeError):
 
 class rb_opts(ctypes.Structure):
     _fields_ = [("schedule", ctypes.c_int), ("device_id", ctypes.c_int),
-                ("reserved", ctypes.c_int * 6)]
+                ("fused_waves", ctypes.c_int), ("no_graph", ctypes.c_int),
+                ("reserved", ctypes.c_int * 4)]
 
 
 class rb_plan_info(ctypes.Structure):

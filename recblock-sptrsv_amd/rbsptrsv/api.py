@@ -49,7 +49,8 @@ class Plan:
     """
 
     def __init__(self, colptr, rowidx, val, substitution=0, lv=-1,
-                 schedule=_lib.SCHED_AUTO, stream=None, device_id=0):
+                 schedule=_lib.SCHED_AUTO, stream=None, device_id=0,
+                 fused_waves=0, graph=True):
         torch = _torch()
         self._keep = []
         dcp = colptr if torch.is_tensor(colptr) else _dev_i32(colptr)
@@ -61,6 +62,8 @@ class Plan:
         lib().rb_opts_default(ctypes.byref(opts))
         opts.schedule = schedule
         opts.device_id = device_id
+        opts.fused_waves = fused_waves
+        opts.no_graph = 0 if graph else 1
         h = ctypes.c_void_p()
         check(lib().rb_plan_create(_ptr(dcp), _ptr(dri), _ptr(dv), self.m,
                                    self.m, self.nnz, substitution, lv,

@@ -283,3 +283,27 @@ def test_legacy_recblocking_solver_host_multi_rhs(oracle, torch):
         set_bench_repeat(old)
     for j in range(rhs):
         assert residual(f, X[j], B[j]) < RES_TOL
+
+
+def test_blocked_graph_replays_match_eager(oracle, torch):
+    """The blocked schedule replays a captured graph of its block walk; every
+    replay (with eager work in between) must equal the kernel-by-kernel
+    launch sequence bitwise."""
+    f = gen.powerlaw(60000, seed=9)
+    p = _plan(f, 3, schedule=1)
+    q = _plan(f, 3, schedule=1, graph=False)
+    bs = [gen.rhs(f.n, seed=s) for s in (11, 12, 13)]
+    eager = [q.solve(torch.as_tensor(b, device="cuda")).cpu().numpy()
+             for b in bs]
+    q.check()
+    re, part, bnz, an = oracle.full_pipeline(f.colptr, f.rowidx, f.val, f.n, 0, 3)
+    for rep in range(3):
+        for i, b in enumerate(bs):
+            x = p.solve(torch.as_tensor(b, device="cuda")).cpu().numpy()
+            torch.cuda.synchronize()
+            _ = torch.ones(1000, device="cuda").sum().item()  # eager work
+            assert np.array_equal(x.view(np.int64), eager[i].view(np.int64))
+            if rep == 0:
+                xo = oracle.solve_blocks(re, f.n, 3, 0, part, b)
+                assert rel_err(x, xo) < X_TOL
+    p.check()
