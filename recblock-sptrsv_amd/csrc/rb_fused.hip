@@ -1,4 +1,5 @@
-// Fused schedule: the whole solve in one persistent launch.
+// Fused schedule: the whole solve in one persistent launch (plus two
+// fully parallel passes that move b into and x out of solve order).
 //
 // Rows are grouped into chains that each belong to one consumer wave.  A
 // wave solves its rows in level order, one *step* (at most 64 rows of one
@@ -16,18 +17,23 @@
 // wave and most entries on the DPP path, so a level costs a few register
 // operations instead of a trip through the memory hierarchy.
 //
-// One workgroup per consumer wave, plus a resolver wave:
-//   consumer  streams its step records (flags, 1/diag, coefficients, ring
-//             sources), the row ids and b straight into registers several
-//             steps ahead; per step: x = (b - fifo - ring - dpp) / diag,
-//             written to the LDS ring, to the slot-order x array when other
-//             waves read it, and to the caller's x;
-//   resolver  polls the global dependencies in order, many loads in flight,
-//             and hands the consumer one partial sum per (step, lane)
-//             through an LDS FIFO.
+// A solve is three launches:
+//   prologue  writes b[orig] of every slot into the slot's step record;
+//   k_fused   one workgroup per consumer wave, plus a resolver wave:
+//     consumer  streams its step records (flags, 1/diag, b, coefficients,
+//               ring sources) into registers several steps ahead; per step
+//               x = (b - fifo - ring - dpp) / diag, written to the LDS ring
+//               and, contiguously, to the slot-order x array;
+//     resolver  polls the global dependencies in order, many loads in
+//               flight, and hands the consumer one partial sum per
+//               (step, lane) through an LDS FIFO;
+//   epilogue  scatters the slot-order x to the caller's x.
+// Keeping the gathers and scatters by row id out of the consumer matters:
+// a wave's scattered access occupies the compute unit's memory pipeline
+// for hundreds of cycles, which would stall the record stream of the
+// dependency chain; in the parallel passes it costs bandwidth only.
 // The slot-order x array is double buffered: a solve reads and writes one
-// buffer and re-arms the published slots of the other with the sentinel,
-// so a solve is one launch with no preparation pass.
+// buffer and re-arms the published slots of the other with the sentinel.
 // Every wait is bounded: on a timeout the kernel sets a status flag and
 // stops instead of hanging.
 #include <algorithm>
@@ -52,29 +58,41 @@ constexpr int kZeroOff = kR * 64 * 8;  // byte offset of the ring's zero
 
 // record flags (int 0 of a lane's record)
 constexpr int kHasGlobal = 1;      // the row has a global group
-// bits 1..15: group offset (groups of lower lanes in this step)
-constexpr int kPublish = 1 << 30;  // other waves read this x
+// word 1: group offset (groups of lower lanes in this step); lane 63's
+// offset plus its own group is the step's group count
 
-// Per-lane record of one step, as 32-bit words:
-//   [0] flags  [1..2] 1/diag
-//   NDPP: [3..4] coefficient of lane l (previous step), [5..6] of lane l-1,
-//         [7..8] of lane l+1
-//   NRING ring sources (byte offsets into the ring), NRING coefficients
+// Per-lane record of one step, as 32-bit words; every double starts at an
+// even word so it is an aligned register pair of the load that brings it:
+//   [0] flags  [1] group offset  [2..3] 1/diag  [4..5] b (per solve)
+//   NDPP: [6..7] coefficient of lane l (previous step), [8..9] of lane l-1,
+//         [10..11] of lane l+1
+//   NRING ring sources (byte offsets into the ring), padded to even, then
+//   NRING coefficients
 // padded to whole 16-byte words; word k of step s for lane l lives at
 // rec[(s * K + k) * 64 + l].
+constexpr int kBWord = 1;  // word holding b (ints 4..5 = .x, .y)
+
+// record prefetch distance in steps for K words per lane (registers:
+// D * 4K); a wave's step count is padded to a multiple of it
+constexpr int prefetch_steps(int K) {
+    return K <= 2 ? 16 : (K <= 3 ? 12 : (K <= 5 ? 8 : (K <= 6 ? 6 : (K <= 8 ? 5 : 4))));
+}
+
 template <int NDPP, int NRING>
 struct Fmt {
-    static constexpr int kSrc = 3 + (NDPP ? 6 : 0);
-    static constexpr int kVal = kSrc + NRING;
+    static constexpr int kSrc = 6 + (NDPP ? 6 : 0);
+    static constexpr int kVal = kSrc + ((NRING + 1) & ~1);
     static constexpr int kInts = kVal + 2 * NRING;
     static constexpr int K = (kInts + 3) / 4;
-    // prefetch distance in steps (registers: D * (4K + 2) + 2D)
-    static constexpr int D =
-        K <= 1 ? 16 : (K <= 3 ? 10 : (K <= 4 ? 8 : (K <= 6 ? 6 : (K <= 7 ? 5 : 4))));
+    static constexpr int D = prefetch_steps(K);
 };
 
 __device__ inline int lds_load(const int *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+// the same counter as a wave-uniform (scalar) value
+__device__ inline int lds_load_u(const int *p) {
+    return __builtin_amdgcn_readfirstlane(lds_load(p));
 }
 __device__ inline void lds_store(int *p, int v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -107,19 +125,16 @@ struct FusedArgs {
     const int *step0;    // [W+1] first step of each wave
     const int *gent0;    // [W+1] first global entry of each wave
     const int *pub0;     // [W+1] first published slot of each wave
-    const int4 *rec;     // [nsteps * K * 64] step records
-    const int *orig;     // [nsteps * 64] original row of each slot, -1 none
-    const double *b;     // caller's b (original order)
+    const int4 *rec;     // [nsteps * K * 64] step records (b filled in)
     const int *gsrc;     // [nent] slot of the dependency
     const double *gval;  // [nent]
     const int *gmeta;    // [nent] group ordinal * 2 + last-of-group
     const int *pub;      // [npub] published slots, grouped by wave
     double *xq;          // [nsteps * 64] x in slot order (this solve)
     double *xq_next;     // the other buffer, re-armed for the next solve
-    double *x;           // caller's x (original order)
     int *status;
     // optional trace: per workgroup 16 counters of wall-clock ticks
-    // (10 ns): [1] consumer FIFO wait, [2] consumer total, [3] steps,
+    // (10 ns): [1] consumer waits, [2] consumer total, [3] steps,
     // [7] resolver FIFO-space wait, [8] resolver poll, [9] batches,
     // [10] poll rounds
     unsigned long long *trace;
@@ -141,35 +156,34 @@ __device__ inline double dpp_from_upper(double v) {
 }
 
 struct Shared {
-    int ready;  // groups resolved (resolver -> consumer)
-    int cons;   // groups consumed (consumer -> resolver)
+    int ready;    // groups resolved (resolver -> consumer)
+    int cons;     // groups consumed (consumer -> resolver)
     int abort;
 };
 
 // Consumer wave.  The step body is instantiated once per position U of a
-// block of DO steps, so every pipeline register index is a compile-time
-// constant and the prefetch buffers stay in registers.
+// block of D steps, so every pipeline register index is a compile-time
+// constant and the record prefetch stays in registers.  A wave's step count
+// is a multiple of D, so blocks have no exit: the loop latch then sees one
+// register assignment of the prefetch buffers, and no copies of loads in
+// flight (which would wait for them) are needed there.
 template <int NDPP, int NRING>
 struct Consumer {
     using F = Fmt<NDPP, NRING>;
-    static constexpr int K = F::K, D = F::D, DO = 2 * D;
+    static constexpr int K = F::K, D = F::D;
 
     const FusedArgs &a;
     const int lane, ns, last;
     const int4 *rec;
-    const int *orp;
     double *xq;
     double *xring;
     const double *fifo;
     Shared &sh;
-    // register pipeline: row ids DO steps ahead, records and b D steps ahead
-    int ob[DO];
-    int4 rb[D][K];
-    double bb[D];
+    int4 rb[D][K];  // records of the next D steps
     int fifo_pos = 0, fifo_avail = 0, sig = 0;
     double xprev = 0.0;
-    bool ok = true, go = true;
-    unsigned long long t_fifo = 0;
+    bool ok = true;
+    unsigned long long t_wait = 0;
 
     __device__ __forceinline__ Consumer(const FusedArgs &a_, int w, int lane_,
                                         double *xring_, const double *fifo_,
@@ -177,7 +191,6 @@ struct Consumer {
         : a(a_), lane(lane_), ns(a_.step0[w + 1] - a_.step0[w]),
           last(a_.step0[w + 1] - a_.step0[w] - 1),
           rec(a_.rec + (size_t)a_.step0[w] * K * 64 + lane_),
-          orp(a_.orig + (size_t)a_.step0[w] * 64 + lane_),
           xq(a_.xq + (size_t)a_.step0[w] * 64 + lane_), xring(xring_),
           fifo(fifo_), sh(sh_) {}
 
@@ -187,72 +200,66 @@ struct Consumer {
         for (int k = 0; k < K; k++) dst[k] = p[k * 64];
     }
 
+    // Records of the first block, oldest first: the loop's wait for block
+    // position U counts the loads issued after U's, and the entry state must
+    // give the same count as the loop itself.
     template <int U>
     __device__ __forceinline__ void prime() {
-        if constexpr (U < DO) {
-            ob[U] = orp[(size_t)min(U, last) * 64];
+        if constexpr (U < D) {
+            load_rec(rb[U], U);
+            __builtin_amdgcn_sched_barrier(0);
             prime<U + 1>();
         }
     }
-    template <int U>
-    __device__ __forceinline__ void prime_rec() {
-        if constexpr (U < D) {
-            load_rec(rb[U], U);
-            bb[U] = (U < ns && ob[U] >= 0) ? a.b[ob[U]] : 0.0;
-            prime_rec<U + 1>();
+
+    // wait until the resolver has published `need` groups; on a timeout or
+    // an abort elsewhere, mark the workgroup aborted (the remaining steps
+    // of the block then run through without waiting)
+    __device__ __forceinline__ void wait_fifo(int need) {
+        const unsigned long long tw = a.trace ? now() : 0;
+        unsigned spins = 0;
+#pragma nounroll
+        while ((fifo_avail = lds_load_u(&sh.ready)) < need) {
+            if (++spins > kSpin || lds_load_u(&sh.abort)) {
+                if (ok && lane == 0) {
+                    atomicOr(a.status, 2);
+                    lds_store(&sh.abort, 1);
+                }
+                ok = false;
+                fifo_avail = 1 << 30;
+                return;
+            }
+            __builtin_amdgcn_s_sleep(1);
         }
+        wg_acquire();
+        if (a.trace) t_wait += now() - tw;
     }
 
     template <int U>
     __device__ __forceinline__ void step(int sb) {
-        if constexpr (U < DO) {
-            constexpr int ur = U % D;
+        if constexpr (U < D) {
             const int s = sb + U;
-            if (s >= ns) {
-                go = false;
-                return;
-            }
-            const int orig = ob[U];
+            // progress counters are wave-uniform; keep them scalar
+            fifo_pos = __builtin_amdgcn_readfirstlane(fifo_pos);
+            fifo_avail = __builtin_amdgcn_readfirstlane(fifo_avail);
+            sig = __builtin_amdgcn_readfirstlane(sig);
             int f[4 * K];
 #pragma unroll
             for (int k = 0; k < K; k++) {
-                f[4 * k] = rb[ur][k].x;
-                f[4 * k + 1] = rb[ur][k].y;
-                f[4 * k + 2] = rb[ur][k].z;
-                f[4 * k + 3] = rb[ur][k].w;
+                f[4 * k] = rb[U][k].x;
+                f[4 * k + 1] = rb[U][k].y;
+                f[4 * k + 2] = rb[U][k].z;
+                f[4 * k + 3] = rb[U][k].w;
             }
-            const double bv = bb[ur];
-            // refill: row ids of step s + DO, record and b of step s + D
-            ob[U] = orp[(size_t)min(s + DO, last) * 64];
-            {
-                const int t = s + D;
-                const int o = ob[(U + D) % DO];
-                load_rec(rb[ur], t);
-                bb[ur] = (t < ns && o >= 0) ? a.b[o] : 0.0;
-            }
-
             const int flags = f[0];
-            const double invd = dbl(f[1], f[2]);
+            const double invd = dbl(f[2], f[3]);
+            const double bv = dbl(f[4], f[5]);
             double acc = 0.0;
-            const int fl = __builtin_amdgcn_readlane(flags, 63);
-            const int gtot = ((fl >> 1) & 0x7fff) + (fl & kHasGlobal);
+            const int gtot = __builtin_amdgcn_readlane(f[1], 63) +
+                             (__builtin_amdgcn_readlane(flags, 63) & kHasGlobal);
             if (gtot > 0) {
-                if (fifo_avail < fifo_pos + gtot) {
-                    const unsigned long long tw = a.trace ? now() : 0;
-                    unsigned spins = 0;
-                    while ((fifo_avail = lds_load(&sh.ready)) < fifo_pos + gtot) {
-                        if (++spins > kSpin || lds_load(&sh.abort)) break;
-                        __builtin_amdgcn_s_sleep(1);
-                    }
-                    if (fifo_avail < fifo_pos + gtot) {
-                        ok = go = false;
-                        return;
-                    }
-                    wg_acquire();
-                    if (a.trace) t_fifo += now() - tw;
-                }
-                const double fv =
-                    fifo[(fifo_pos + ((flags >> 1) & 0x7fff)) & (kFF - 1)];
+                if (fifo_avail < fifo_pos + gtot) wait_fifo(fifo_pos + gtot);
+                const double fv = fifo[(fifo_pos + f[1]) & (kFF - 1)];
                 acc = (flags & kHasGlobal) ? fv : 0.0;
                 fifo_pos += gtot;
             }
@@ -265,20 +272,25 @@ struct Consumer {
             if constexpr (NDPP > 0) {
                 const double xl = dpp_from_lower(xprev);
                 const double xr = dpp_from_upper(xprev);
-                acc += dbl(f[3], f[4]) * xprev;
-                acc += dbl(f[5], f[6]) * xl;
-                acc += dbl(f[7], f[8]) * xr;
+                acc += dbl(f[6], f[7]) * xprev;
+                acc += dbl(f[8], f[9]) * xl;
+                acc += dbl(f[10], f[11]) * xr;
             }
             const double xv = (bv - acc) * invd;
             xprev = xv;
             if constexpr (NRING > 0) xring[(s & (kR - 1)) * 64 + lane] = xv;
-            if (orig >= 0) a.x[orig] = xv;
-            if (flags & kPublish) st_x(&xq[(size_t)s * 64], xv);
-            if (fifo_pos - sig >= kSigLag || (U == DO - 1 && fifo_pos != sig)) {
+            // every slot goes out (the epilogue reads them); agent scope so
+            // that published slots are visible to the other waves' resolvers
+            st_x(&xq[(size_t)s * 64], xv);
+            if ((U == D - 1 && fifo_pos != sig) || fifo_pos - sig >= kSigLag) {
                 wg_release();
                 if (lane == 0) lds_store(&sh.cons, fifo_pos);
                 sig = fifo_pos;
             }
+            // refill after the last use of this record: the new load can
+            // take the same registers
+            __builtin_amdgcn_sched_barrier(0);
+            load_rec(rb[U], s + D);
             step<U + 1>(sb);
         }
     }
@@ -286,16 +298,11 @@ struct Consumer {
     __device__ __forceinline__ void run(int w) {
         const unsigned long long t0 = a.trace ? now() : 0;
         prime<0>();
-        prime_rec<0>();
-        for (int sb = 0; go; sb += DO) step<0>(sb);
-        if (!ok && lane == 0) {
-            atomicOr(a.status, 2);
-            lds_store(&sh.abort, 1);
-        }
+        for (int sb = 0; sb < ns && ok; sb += D) step<0>(sb);
         if (lane == 0) lds_store(&sh.cons, 1 << 30);
         if (a.trace && lane == 0) {
             unsigned long long *T = a.trace + (size_t)w * 16;
-            T[1] = t_fifo;
+            T[1] = t_wait;
             T[2] = now() - t0;
             T[3] = ns;
         }
@@ -339,8 +346,9 @@ __device__ __forceinline__ void resolve(const FusedArgs &a, int w, int lane,
             const int nt = min(64 * kU, e1 - kb);
             const int gmax = a.gmeta[kb + nt - 1] >> 1;
             unsigned spins = 0;
-            while (gmax - lds_load(&sh.cons) >= kFF) {
-                if (++spins > kSpin || lds_load(&sh.abort)) {
+#pragma nounroll
+            while (gmax - lds_load_u(&sh.cons) >= kFF) {
+                if (++spins > kSpin || lds_load_u(&sh.abort)) {
                     abort = true;
                     break;
                 }
@@ -418,7 +426,7 @@ __device__ __forceinline__ void resolve(const FusedArgs &a, int w, int lane,
                 if (fp >= n) break;
                 nround++;
                 ++spins;
-                if (spins > kSpin || lds_load(&sh.abort) ||
+                if (spins > kSpin || lds_load_u(&sh.abort) ||
                     ((spins & 1023) == 0 && status_now(a.status))) {
                     if (lane == 0) {
                         atomicOr(a.status, 4);
@@ -436,7 +444,7 @@ __device__ __forceinline__ void resolve(const FusedArgs &a, int w, int lane,
             }
         }
         if (a.trace) t_poll += now() - tw;
-        if (lds_load(&sh.abort)) abort = true;
+        if (lds_load_u(&sh.abort)) abort = true;
     }
     if (a.trace && lane == 0) {
         unsigned long long *T = a.trace + (size_t)w * 16;
@@ -457,7 +465,7 @@ __global__ void __launch_bounds__(128) k_fused(FusedArgs a) {
     __shared__ Shared sh;
 
     const int w = blockIdx.x;
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (threadIdx.x == 0) {
         sh.ready = 0;
@@ -469,8 +477,30 @@ __global__ void __launch_bounds__(128) k_fused(FusedArgs a) {
     if (wave == 0) {
         Consumer<NDPP, NRING> c(a, w, lane, xring, fifo, sh);
         c.run(w);
-    } else
+    } else {
         resolve(a, w, lane, fifo, rtmp, gtmp, &ctmp, sh);
+    }
+}
+
+// b of every slot into its step record (0 for padding lanes)
+__global__ void k_fused_prologue(const int *__restrict__ orig, long nslots,
+                                 int K, const double *__restrict__ b,
+                                 int4 *rec) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nslots) return;
+    const int o = orig[i];
+    const double v = o >= 0 ? b[o] : 0.0;
+    int2 *p = (int2 *)&rec[((i >> 6) * K + kBWord) * 64 + (i & 63)];
+    *p = make_int2(__double2loint(v), __double2hiint(v));
+}
+
+// slot-order x to the caller's x
+__global__ void k_fused_epilogue(const int *__restrict__ orig, long nslots,
+                                 const double *__restrict__ xq, double *x) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nslots) return;
+    const int o = orig[i];
+    if (o >= 0) x[o] = xq[i];
 }
 
 __global__ void k_fill_pending(double *p, long n) {
@@ -498,7 +528,7 @@ struct FusedPlan {
 static int ring_class(int e) { return e == 0 ? 0 : (e <= 2 ? 2 : (e <= 4 ? 4 : 8)); }
 
 static int words_for(int ndpp, int nring) {
-    const int ints = 3 + (ndpp ? 6 : 0) + 3 * nring;
+    const int ints = 6 + (ndpp ? 6 : 0) + ((nring + 1) & ~1) + 2 * nring;
     return (ints + 3) / 4;
 }
 
@@ -654,7 +684,15 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
     }
     const int NRING = ring_class(std::min(emax, 8));
     const int K = words_for(NDPP, NRING);
-    const int kSrc = 3 + (NDPP ? 6 : 0), kVal = kSrc + NRING;
+    // pad every wave to whole prefetch blocks with empty steps
+    {
+        const int Dp = prefetch_steps(K);
+        std::array<int, 64> empty;
+        empty.fill(-1);
+        for (int w = 0; w < W; w++)
+            while (steps[w].size() % Dp) steps[w].push_back(empty);
+    }
+    const int kSrc = 6 + (NDPP ? 6 : 0), kVal = kSrc + ((NRING + 1) & ~1);
     // ---- emit records
     std::vector<int> step0(W + 1, 0), gent0(W + 1, 0), pub0(W + 1, 0);
     for (int w = 0; w < W; w++) step0[w + 1] = step0[w] + (int)steps[w].size();
@@ -685,7 +723,7 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
                 const int q = st[l];
                 std::fill(f.begin(), f.end(), 0);
                 for (int j = 0; j < NRING; j++) f[kSrc + j] = kZeroOff;
-                int flags = gofs << 1;
+                int flags = 0;
                 if (q >= 0) {
                     const int r = pof(q);
                     origv[(size_t)(step0[w] + si) * 64 + l] = lf[r];
@@ -696,7 +734,7 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
                         const int u = qof(cc[e]);
                         if (NDPP && is_dpp(q, u)) {
                             const int du = lanew[u] - l;  // -1, 0, +1
-                            put(du == 0 ? 3 : (du < 0 ? 5 : 7), cv[e]);
+                            put(du == 0 ? 6 : (du < 0 ? 8 : 10), cv[e]);
                         } else if (is_loc(q, u) && nl < NRING) {
                             f[kSrc + nl] = ((stepw[u] % kR) * 64 + lanew[u]) * 8;
                             put(kVal + 2 * nl, cv[e]);
@@ -712,12 +750,13 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
                     if (hasg) {
                         gmeta.back() |= 1;
                         gord++;
-                        flags = (gofs << 1) | kHasGlobal;
-                        gofs++;
+                        flags = kHasGlobal;
                     }
-                    put(1, invd[r]);
+                    put(2, invd[r]);
                 }
                 f[0] = flags;
+                f[1] = gofs;
+                gofs += flags & kHasGlobal;
                 for (int k = 0; k < K; k++)
                     rec[sbase + (size_t)k * 64 + l] =
                         make_int4(f[4 * k], f[4 * k + 1], f[4 * k + 2], f[4 * k + 3]);
@@ -730,7 +769,6 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
     for (int q = 0; q < m; q++)
         if (needpub[q]) {
             const int sl = slot_of(q);
-            rec[(size_t)(sl / 64) * K * 64 + (sl % 64)].x |= kPublish;
             pubw[wave[q]].push_back(sl);
         }
     std::vector<int> pub;
@@ -866,11 +904,17 @@ rb_status solve_fused(rb_plan *p, const double *d_b, int ldb, double *d_x,
     for (int k = 0; k < nrhs; k++) {
         double *cur = F->d_xq + (F->epoch & 1) * nslots;
         double *nxt = F->d_xq + ((F->epoch + 1) & 1) * nslots;
+        const int grid = (int)((nslots + 255) / 256);
+        k_fused_prologue<<<grid, 256, 0, s>>>(F->d_orig, (long)nslots, F->K,
+                                              d_b + (size_t)k * ldb, F->d_rec);
+        RB_HIP(hipGetLastError());
         FusedArgs args{F->d_step0, F->d_gent0, F->d_pub0, F->d_rec,
-                       F->d_orig, d_b + (size_t)k * ldb, F->d_gsrc,
-                       F->d_gval, F->d_gmeta, F->d_pub, cur, nxt,
-                       d_x + (size_t)k * ldx, p->d_status, F->d_trace};
+                       F->d_gsrc, F->d_gval, F->d_gmeta, F->d_pub, cur, nxt,
+                       p->d_status, F->d_trace};
         RB_TRY(F->NDPP ? launch_ring<3>(F, args, s) : launch_ring<0>(F, args, s));
+        k_fused_epilogue<<<grid, 256, 0, s>>>(F->d_orig, (long)nslots, cur,
+                                              d_x + (size_t)k * ldx);
+        RB_HIP(hipGetLastError());
         F->epoch++;
     }
     return RB_OK;
