@@ -178,14 +178,23 @@ rb_status rb_plan_block_csr(const rb_plan *plan, int i, int *rowptr,
 
 /* Trace of the fused schedule's last solve, recorded when the environment
  * variable RB_FUSED_TRACE is set at plan build: 16 counters per consumer
- * wave (wall-clock ticks of 10 ns): consumer waiting for records, waiting
- * for global dependencies, total, steps; feeder waiting for ring space,
- * waiting for its loads, chunks; resolver waiting for FIFO space, polling,
- * batches, re-poll rounds.  *nwaves receives the wave count (0 when the
- * plan has no fused schedule).  counters may be NULL.  No counterpart in
- * the reference (which times whole solves only). */
+ * wave (wall-clock ticks of 10 ns): [1] consumer waiting for global
+ * dependencies, [2] consumer total, [3] steps, [7] resolver waiting for
+ * FIFO space, [8] polling, [9] batches, [10] re-poll rounds (others 0).
+ * *nwaves receives the wave count (0 when the plan has no fused schedule).
+ * counters may be NULL.  No counterpart in the reference (which times
+ * whole solves only). */
 rb_status rb_plan_trace(const rb_plan *plan, unsigned long long *counters,
                         int *nwaves);
+
+/* Kernel timing for measurement harnesses.  While enabled, every solve
+ * records HIP events on its stream around its main launch: the fused
+ * kernel (fused schedule) or the captured block walk (blocked schedule),
+ * without the passes that move b and x between orders.
+ * rb_plan_kernel_ms waits for the last such launch and returns its
+ * duration in milliseconds.  No counterpart in the reference. */
+rb_status rb_plan_set_timing(rb_plan *plan, int enable);
+rb_status rb_plan_kernel_ms(rb_plan *plan, double *ms);
 
 /* The level-ordered (permuted) matrix as CSC, host arrays of m+1 / nnz. */
 rb_status rb_plan_permuted_csc(const rb_plan *plan, int *colptr, int *rowidx,

@@ -215,9 +215,40 @@ rb_status rb_plan_check(rb_plan *p, void *stream) {
     return ok();
 }
 
+rb_status rb_plan_set_timing(rb_plan *p, int enable) {
+    if (!p) {
+        set_error(RB_ERR_INVALID_ARG, "null plan");
+        return RB_ERR_INVALID_ARG;
+    }
+    if (enable && !p->ev0) {
+        RB_HIP(hipEventCreate(&p->ev0));
+        RB_HIP(hipEventCreate(&p->ev1));
+    }
+    p->timing = enable != 0;
+    return ok();
+}
+
+rb_status rb_plan_kernel_ms(rb_plan *p, double *ms) {
+    if (!p || !ms) {
+        set_error(RB_ERR_INVALID_ARG, "null argument");
+        return RB_ERR_INVALID_ARG;
+    }
+    if (!p->ev0) {
+        set_error(RB_ERR_INVALID_ARG, "timing was never enabled on this plan");
+        return RB_ERR_INVALID_ARG;
+    }
+    RB_HIP(hipEventSynchronize(p->ev1));
+    float f = 0.0f;
+    RB_HIP(hipEventElapsedTime(&f, p->ev0, p->ev1));
+    *ms = f;
+    return ok();
+}
+
 rb_status rb_plan_destroy(rb_plan *p) {
     if (!p) return ok();
     (void)hipDeviceSynchronize();
+    if (p->ev0) (void)hipEventDestroy(p->ev0);
+    if (p->ev1) (void)hipEventDestroy(p->ev1);
     destroy_graph(p);
     fused_destroy(p);
     p->arena.release();

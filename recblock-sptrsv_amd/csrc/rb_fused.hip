@@ -911,7 +911,9 @@ rb_status solve_fused(rb_plan *p, const double *d_b, int ldb, double *d_x,
         FusedArgs args{F->d_step0, F->d_gent0, F->d_pub0, F->d_rec,
                        F->d_gsrc, F->d_gval, F->d_gmeta, F->d_pub, cur, nxt,
                        p->d_status, F->d_trace};
+        mark_start(p, s);
         RB_TRY(F->NDPP ? launch_ring<3>(F, args, s) : launch_ring<0>(F, args, s));
+        mark_end(p, s);
         k_fused_epilogue<<<grid, 256, 0, s>>>(F->d_orig, (long)nslots, cur,
                                               d_x + (size_t)k * ldx);
         RB_HIP(hipGetLastError());

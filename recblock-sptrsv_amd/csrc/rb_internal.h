@@ -145,6 +145,9 @@ struct rb_plan {
     // blocked schedule: the block walk captured once as a graph
     void *graph_exec = nullptr;
     bool graph_failed = false;
+    // kernel timing (rb_plan_set_timing)
+    bool timing = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
 namespace rb {
@@ -158,6 +161,13 @@ rb_status build_fused(rb_plan *p, const int *d_prptr, const int *d_ccol,
 rb_status solve_fused(rb_plan *p, const double *d_b, int ldb, double *d_x,
                       int ldx, int nrhs, hipStream_t s);
 void fused_destroy(rb_plan *p);
+// events around a solve's main launch when timing is on
+inline void mark_start(rb_plan *p, hipStream_t s) {
+    if (p->timing) (void)hipEventRecord(p->ev0, s);
+}
+inline void mark_end(rb_plan *p, hipStream_t s) {
+    if (p->timing) (void)hipEventRecord(p->ev1, s);
+}
 void destroy_graph(rb_plan *p);
 int fused_max_waves();
 }  // namespace rb

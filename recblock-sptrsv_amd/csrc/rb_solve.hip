@@ -365,10 +365,12 @@ rb_status solve_blocked(rb_plan *p, const double *d_b, int ldb, double *d_x,
         const int pro = nblk(m > p->ntri ? m : p->ntri);
         k_prologue<<<pro, kT, 0, s>>>(b, p->d_levelItem, m, p->d_bperm,
                                       p->d_xperm, p->d_tickets, p->ntri);
+        mark_start(p, s);
         if (graph)
             RB_HIP(hipGraphLaunch((hipGraphExec_t)p->graph_exec, s));
         else
             launch_blocks(p, s);
+        mark_end(p, s);
         k_epilogue<<<nblk(m), kT, 0, s>>>(p->d_xperm, p->d_levelItem, m, x);
         RB_HIP(hipGetLastError());
     }

@@ -20,6 +20,7 @@ EXPORTS = [
     "rb_plan_info_get",
     "rb_plan_blocks", "rb_plan_levels", "rb_plan_tri_levels",
     "rb_plan_block_csr", "rb_plan_permuted_csc", "rb_plan_trace",
+    "rb_plan_set_timing", "rb_plan_kernel_ms",
 ]
 
 RB_OK = 0

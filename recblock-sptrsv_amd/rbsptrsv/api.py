@@ -121,6 +121,17 @@ class Plan:
                              _stream_ptr(stream)))
         return x
 
+    def set_timing(self, enable=True):
+        """Record events around each solve's main launch (see
+        rb_plan_set_timing)."""
+        check(lib().rb_plan_set_timing(self._h, 1 if enable else 0))
+
+    def kernel_ms(self):
+        """Duration of the last timed main launch, in ms."""
+        ms = ctypes.c_double(0.0)
+        check(lib().rb_plan_kernel_ms(self._h, ctypes.byref(ms)))
+        return ms.value
+
     def check(self, stream=None):
         """Synchronise and raise if the device watchdog fired."""
         check(lib().rb_plan_check(self._h, _stream_ptr(stream)))

@@ -307,3 +307,20 @@ def test_blocked_graph_replays_match_eager(oracle, torch):
                 xo = oracle.solve_blocks(re, f.n, 3, 0, part, b)
                 assert rel_err(x, xo) < X_TOL
     p.check()
+
+
+@pytest.mark.parametrize("schedule", [1, 2])
+def test_kernel_timing_events(torch, schedule):
+    f = gen.laplacian(60, 50)
+    p = _plan(f, 2, schedule=schedule)
+    b = torch.as_tensor(gen.rhs(f.n, 2), device="cuda")
+    from rbsptrsv import RbError
+    with pytest.raises(RbError):
+        p.kernel_ms()                       # never enabled
+    p.set_timing(True)
+    x = p.solve(b)
+    ms = p.kernel_ms()
+    assert 0.0 < ms < 1000.0
+    p.set_timing(False)
+    x2 = p.solve(b)
+    assert np.array_equal(x.cpu().numpy(), x2.cpu().numpy())
