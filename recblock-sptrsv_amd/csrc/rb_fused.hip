@@ -130,8 +130,11 @@ struct FusedArgs {
     const double *gval;  // [nent]
     const int *gmeta;    // [nent] group ordinal * 2 + last-of-group
     const int *pub;      // [npub] published slots, grouped by wave
-    double *xq;          // [nsteps * 64] x in slot order (this solve)
-    double *xq_next;     // the other buffer, re-armed for the next solve
+    double *xq;          // [nr][nsteps * 64] x in slot order (this solve)
+    double *xq_next;     // the other buffers, re-armed for the next solve
+    const double *brec;  // [nr][nsteps * 64] b in slot order (SEPB)
+    long nslots;         // nsteps * 64: stride between right-hand sides
+    int W;               // consumer waves per right-hand side
     int *status;
     // optional trace: per workgroup 16 counters of wall-clock ticks
     // (10 ns): [1] consumer waits, [2] consumer total, [3] steps,
@@ -167,7 +170,9 @@ struct Shared {
 // is a multiple of D, so blocks have no exit: the loop latch then sees one
 // register assignment of the prefetch buffers, and no copies of loads in
 // flight (which would wait for them) are needed there.
-template <int NDPP, int NRING>
+// SEPB: b comes from a separate slot-order array (several right-hand
+// sides per launch share the records) instead of the record's b word.
+template <int NDPP, int NRING, bool SEPB>
 struct Consumer {
     using F = Fmt<NDPP, NRING>;
     static constexpr int K = F::K, D = F::D;
@@ -176,28 +181,34 @@ struct Consumer {
     const int lane, ns, last;
     const int4 *rec;
     double *xq;
+    const double *br;
     double *xring;
     const double *fifo;
     Shared &sh;
     int4 rb[D][K];  // records of the next D steps
+    double bb[SEPB ? D : 1];  // b of the next D steps (SEPB)
     int fifo_pos = 0, fifo_avail = 0, sig = 0;
     double xprev = 0.0;
     bool ok = true;
     unsigned long long t_wait = 0;
 
-    __device__ __forceinline__ Consumer(const FusedArgs &a_, int w, int lane_,
-                                        double *xring_, const double *fifo_,
-                                        Shared &sh_)
+    __device__ __forceinline__ Consumer(const FusedArgs &a_, int w, int r,
+                                        int lane_, double *xring_,
+                                        const double *fifo_, Shared &sh_)
         : a(a_), lane(lane_), ns(a_.step0[w + 1] - a_.step0[w]),
           last(a_.step0[w + 1] - a_.step0[w] - 1),
           rec(a_.rec + (size_t)a_.step0[w] * K * 64 + lane_),
-          xq(a_.xq + (size_t)a_.step0[w] * 64 + lane_), xring(xring_),
-          fifo(fifo_), sh(sh_) {}
+          xq(a_.xq + r * a_.nslots + (size_t)a_.step0[w] * 64 + lane_),
+          br(SEPB ? a_.brec + r * a_.nslots + (size_t)a_.step0[w] * 64 + lane_
+                  : nullptr),
+          xring(xring_), fifo(fifo_), sh(sh_) {}
 
-    __device__ __forceinline__ void load_rec(int4 (&dst)[K], int t) {
-        const int4 *p = rec + (size_t)min(t, last) * K * 64;
+    __device__ __forceinline__ void load_rec(int U, int t) {
+        const size_t tc = (size_t)min(t, last);
+        const int4 *p = rec + tc * K * 64;
 #pragma unroll
-        for (int k = 0; k < K; k++) dst[k] = p[k * 64];
+        for (int k = 0; k < K; k++) rb[U][k] = p[k * 64];
+        if constexpr (SEPB) bb[U] = br[tc * 64];
     }
 
     // Records of the first block, oldest first: the loop's wait for block
@@ -206,7 +217,7 @@ struct Consumer {
     template <int U>
     __device__ __forceinline__ void prime() {
         if constexpr (U < D) {
-            load_rec(rb[U], U);
+            load_rec(U, U);
             __builtin_amdgcn_sched_barrier(0);
             prime<U + 1>();
         }
@@ -253,7 +264,9 @@ struct Consumer {
             }
             const int flags = f[0];
             const double invd = dbl(f[2], f[3]);
-            const double bv = dbl(f[4], f[5]);
+            double bv;
+            if constexpr (SEPB) bv = bb[U];
+            else bv = dbl(f[4], f[5]);
             double acc = 0.0;
             const int gtot = __builtin_amdgcn_readlane(f[1], 63) +
                              (__builtin_amdgcn_readlane(flags, 63) & kHasGlobal);
@@ -290,17 +303,17 @@ struct Consumer {
             // refill after the last use of this record: the new load can
             // take the same registers
             __builtin_amdgcn_sched_barrier(0);
-            load_rec(rb[U], s + D);
+            load_rec(U, s + D);
             step<U + 1>(sb);
         }
     }
 
-    __device__ __forceinline__ void run(int w) {
+    __device__ __forceinline__ void run(int w, int r) {
         const unsigned long long t0 = a.trace ? now() : 0;
         prime<0>();
         for (int sb = 0; sb < ns && ok; sb += D) step<0>(sb);
         if (lane == 0) lds_store(&sh.cons, 1 << 30);
-        if (a.trace && lane == 0) {
+        if (a.trace && lane == 0 && r == 0) {
             unsigned long long *T = a.trace + (size_t)w * 16;
             T[1] = t_wait;
             T[2] = now() - t0;
@@ -309,13 +322,15 @@ struct Consumer {
     }
 };
 
-__device__ __forceinline__ void resolve(const FusedArgs &a, int w, int lane,
-                                        double *fifo, double *rtmp, int *gtmp,
-                                        double *ctmp, Shared &sh) {
+__device__ __forceinline__ void resolve(const FusedArgs &a, int w, int r,
+                                        int lane, double *fifo, double *rtmp,
+                                        int *gtmp, double *ctmp, Shared &sh) {
+    const double *xq = a.xq + r * a.nslots;
     // re-arm this wave's published slots of the next solve's buffer; no
     // wave of this launch reads that buffer
+    unsigned long long *nxt = (unsigned long long *)(a.xq_next + r * a.nslots);
     for (int i = a.pub0[w] + lane; i < a.pub0[w + 1]; i += 64)
-        ((unsigned long long *)a.xq_next)[a.pub[i]] = kPending;
+        nxt[a.pub[i]] = kPending;
 
     const int e0 = a.gent0[w], e1 = a.gent0[w + 1];
     double carry = 0.0;
@@ -362,7 +377,7 @@ __device__ __forceinline__ void resolve(const FusedArgs &a, int w, int lane,
         }
 #pragma unroll
         for (int u = 0; u < kU; u++)
-            xvu[u] = kb + 64 * u + lane < e1 ? ld_x(&a.xq[srcu[u]]) : 0.0;
+            xvu[u] = kb + 64 * u + lane < e1 ? ld_x(&xq[srcu[u]]) : 0.0;
 #pragma unroll
         for (int u = 0; u < kU; u++) {
             const int k0 = kb + 64 * u;
@@ -437,7 +452,7 @@ __device__ __forceinline__ void resolve(const FusedArgs &a, int w, int lane,
                 }
                 __builtin_amdgcn_s_sleep(1);
                 if (pend) {
-                    xv = ld_x(&a.xq[src]);
+                    xv = ld_x(&xq[src]);
                     pend = pending(xv);
                     if (!pend) rtmp[lane] = gv * xv;
                 }
@@ -446,7 +461,7 @@ __device__ __forceinline__ void resolve(const FusedArgs &a, int w, int lane,
         if (a.trace) t_poll += now() - tw;
         if (lds_load_u(&sh.abort)) abort = true;
     }
-    if (a.trace && lane == 0) {
+    if (a.trace && lane == 0 && r == 0) {
         unsigned long long *T = a.trace + (size_t)w * 16;
         T[7] = t_rspace;
         T[8] = t_poll;
@@ -455,7 +470,9 @@ __device__ __forceinline__ void resolve(const FusedArgs &a, int w, int lane,
     }
 }
 
-template <int NDPP, int NRING>
+// One workgroup per (consumer wave w, right-hand side r): blockIdx.x =
+// r * W + w.  Right-hand sides share the records and nothing else.
+template <int NDPP, int NRING, bool SEPB>
 __global__ void __launch_bounds__(128) k_fused(FusedArgs a) {
     __shared__ double xring[NRING > 0 ? kR * 64 + 1 : 1];
     __shared__ double fifo[kFF];
@@ -464,7 +481,7 @@ __global__ void __launch_bounds__(128) k_fused(FusedArgs a) {
     __shared__ double ctmp;
     __shared__ Shared sh;
 
-    const int w = blockIdx.x;
+    const int w = blockIdx.x % a.W, r = blockIdx.x / a.W;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (threadIdx.x == 0) {
@@ -475,10 +492,10 @@ __global__ void __launch_bounds__(128) k_fused(FusedArgs a) {
     }
     __syncthreads();
     if (wave == 0) {
-        Consumer<NDPP, NRING> c(a, w, lane, xring, fifo, sh);
-        c.run(w);
+        Consumer<NDPP, NRING, SEPB> c(a, w, r, lane, xring, fifo, sh);
+        c.run(w, r);
     } else {
-        resolve(a, w, lane, fifo, rtmp, gtmp, &ctmp, sh);
+        resolve(a, w, r, lane, fifo, rtmp, gtmp, &ctmp, sh);
     }
 }
 
@@ -494,13 +511,27 @@ __global__ void k_fused_prologue(const int *__restrict__ orig, long nslots,
     *p = make_int2(__double2loint(v), __double2hiint(v));
 }
 
-// slot-order x to the caller's x
-__global__ void k_fused_epilogue(const int *__restrict__ orig, long nslots,
-                                 const double *__restrict__ xq, double *x) {
+// b of nr right-hand sides in slot order (0 for padding lanes)
+__global__ void k_fused_prologue_multi(const int *__restrict__ orig,
+                                       long nslots, int nr,
+                                       const double *__restrict__ b, long ldb,
+                                       double *brec) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nslots) return;
-    const int o = orig[i];
-    if (o >= 0) x[o] = xq[i];
+    if (i >= nslots * nr) return;
+    const long r = i / nslots, q = i - r * nslots;
+    const int o = orig[q];
+    brec[i] = o >= 0 ? b[r * ldb + o] : 0.0;
+}
+
+// slot-order x of nr right-hand sides to the caller's x
+__global__ void k_fused_epilogue(const int *__restrict__ orig, long nslots,
+                                 int nr, const double *__restrict__ xq,
+                                 double *x, long ldx) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nslots * nr) return;
+    const long r = i / nslots, q = i - r * nslots;
+    const int o = orig[q];
+    if (o >= 0) x[r * ldx + o] = xq[i];
 }
 
 __global__ void k_fill_pending(double *p, long n) {
@@ -520,9 +551,20 @@ struct FusedPlan {
     int *d_orig = nullptr;
     int *d_gsrc = nullptr, *d_gmeta = nullptr, *d_pub = nullptr;
     double *d_gval = nullptr;
-    double *d_xq = nullptr;  // two buffers of nsteps * 64
+    // slot-order x: two buffers of nr_alloc right-hand sides each, and
+    // which (buffer, right-hand side) regions hold the sentinel in every
+    // published slot
+    double *d_xq = nullptr;
+    int nr_alloc = 0;
+    std::vector<char> armed[2];
+    double *d_brec = nullptr;  // b in slot order, nr_alloc right-hand sides
+    int nr_cap = 0;            // right-hand sides per launch (co-residency)
     unsigned long long *d_trace = nullptr;  // RB_FUSED_TRACE=1
     double build_ms = 0.0;
+    ~FusedPlan() {
+        if (d_xq) (void)hipFree(d_xq);
+        if (d_brec) (void)hipFree(d_brec);
+    }
 };
 
 static int ring_class(int e) { return e == 0 ? 0 : (e <= 2 ? 2 : (e <= 4 ? 4 : 8)); }
@@ -531,6 +573,8 @@ static int words_for(int ndpp, int nring) {
     const int ints = 6 + (ndpp ? 6 : 0) + ((nring + 1) & ~1) + 2 * nring;
     return (ints + 3) / 4;
 }
+
+static rb_status ensure_rhs(FusedPlan *F, int nr, hipStream_t s);
 
 // Host partitioner and record builder.  rp/cc/cv: permuted CSR (ascending
 // columns); lp: BFS level boundaries; lf: final permutation; invd: 1/diag.
@@ -803,7 +847,6 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
     RB_TRY(A.alloc(&F->d_gval, gval.size()));
     RB_TRY(A.alloc(&F->d_gmeta, gmeta.size()));
     RB_TRY(A.alloc(&F->d_pub, pub.size()));
-    RB_TRY(A.alloc(&F->d_xq, 2 * nslots));
     if (getenv("RB_FUSED_TRACE")) {
         RB_TRY(A.alloc(&F->d_trace, (size_t)W * 16));
         RB_HIP(hipMemsetAsync(F->d_trace, 0, sizeof(unsigned long long) * W * 16, s));
@@ -821,17 +864,12 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
     RB_TRY(up(F->d_gval, gval.data(), sizeof(double) * gval.size()));
     RB_TRY(up(F->d_gmeta, gmeta.data(), sizeof(int) * gmeta.size()));
     RB_TRY(up(F->d_pub, pub.data(), sizeof(int) * pub.size()));
-    {
-        const long n = 2 * (long)nslots;
-        k_fill_pending<<<(int)((n + 255) / 256), 256, 0, s>>>(F->d_xq, n);
-        RB_HIP(hipGetLastError());
-    }
     RB_HIP(hipStreamSynchronize(s));
     auto t1 = std::chrono::steady_clock::now();
     F->build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     delete p->fused;
     p->fused = F;
-    return RB_OK;
+    return ensure_rhs(F, 1, s);
 }
 
 // Device part of the fused build: pull the permuted CSR and the tables the
@@ -872,24 +910,59 @@ int fused_max_waves() {
     return ncu > 0 ? ncu : 256;
 }
 
-template <int NDPP, int NRING>
-static rb_status launch_fused(FusedPlan *F, const FusedArgs &args,
-                              hipStream_t s) {
-    void *kargs[] = {(void *)&args};
-    hipError_t e = hipLaunchCooperativeKernel(
-        (const void *)k_fused<NDPP, NRING>, dim3(F->W), dim3(128), kargs, 0, s);
-    if (e != hipSuccess) return status_of_hip(e, "hipLaunchCooperativeKernel(k_fused)");
+template <int NDPP, int NRING, bool SEPB>
+static const void *kernel_of() {
+    return (const void *)k_fused<NDPP, NRING, SEPB>;
+}
+
+template <bool SEPB>
+static const void *pick_kernel(const FusedPlan *F) {
+    const int r = F->NRING;
+    if (F->NDPP)
+        return r == 0 ? kernel_of<3, 0, SEPB>() : r == 2 ? kernel_of<3, 2, SEPB>()
+             : r == 4 ? kernel_of<3, 4, SEPB>() : kernel_of<3, 8, SEPB>();
+    return r == 0 ? kernel_of<0, 0, SEPB>() : r == 2 ? kernel_of<0, 2, SEPB>()
+         : r == 4 ? kernel_of<0, 4, SEPB>() : kernel_of<0, 8, SEPB>();
+}
+
+// Slot-order x buffers for nr right-hand sides (grown on demand; a fresh
+// allocation holds the sentinel everywhere).
+static rb_status ensure_rhs(FusedPlan *F, int nr, hipStream_t s) {
+    if (nr <= F->nr_alloc) return RB_OK;
+    const size_t nslots = (size_t)F->nsteps * 64;
+    RB_HIP(hipStreamSynchronize(s));
+    if (F->d_xq) RB_HIP(hipFree(F->d_xq));
+    if (F->d_brec) RB_HIP(hipFree(F->d_brec));
+    F->d_xq = nullptr;
+    F->d_brec = nullptr;
+    F->nr_alloc = 0;
+    if (hipMalloc(&F->d_xq, 2 * nr * nslots * sizeof(double)) != hipSuccess ||
+        (nr > 1 && hipMalloc(&F->d_brec, nr * nslots * sizeof(double)) != hipSuccess)) {
+        (void)hipGetLastError();
+        set_error(RB_ERR_NO_MEMORY, "hipMalloc failed (slot-order x buffers)");
+        return RB_ERR_NO_MEMORY;
+    }
+    const long n = 2L * nr * (long)nslots;
+    k_fill_pending<<<(int)((n + 255) / 256), 256, 0, s>>>(F->d_xq, n);
+    RB_HIP(hipGetLastError());
+    F->nr_alloc = nr;
+    F->armed[0].assign(nr, 1);
+    F->armed[1].assign(nr, 1);
     return RB_OK;
 }
 
-template <int NDPP>
-static rb_status launch_ring(FusedPlan *F, const FusedArgs &args, hipStream_t s) {
-    switch (F->NRING) {
-    case 0: return launch_fused<NDPP, 0>(F, args, s);
-    case 2: return launch_fused<NDPP, 2>(F, args, s);
-    case 4: return launch_fused<NDPP, 4>(F, args, s);
-    default: return launch_fused<NDPP, 8>(F, args, s);
-    }
+// Right-hand sides that fit one cooperative launch (W workgroups each).
+static int rhs_per_launch(FusedPlan *F) {
+    if (F->nr_cap > 0) return F->nr_cap;
+    int dev = 0, ncu = 0, per = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, pick_kernel<true>(F),
+                                                     128, 0) != hipSuccess)
+        per = 1;
+    (void)hipGetLastError();
+    F->nr_cap = std::max(1, std::min(64, (per * ncu) / std::max(1, F->W)));
+    return F->nr_cap;
 }
 
 rb_status solve_fused(rb_plan *p, const double *d_b, int ldb, double *d_x,
@@ -901,22 +974,48 @@ rb_status solve_fused(rb_plan *p, const double *d_b, int ldb, double *d_x,
     }
     if (p->m == 0) return RB_OK;
     const size_t nslots = (size_t)F->nsteps * 64;
-    for (int k = 0; k < nrhs; k++) {
-        double *cur = F->d_xq + (F->epoch & 1) * nslots;
-        double *nxt = F->d_xq + ((F->epoch + 1) & 1) * nslots;
-        const int grid = (int)((nslots + 255) / 256);
-        k_fused_prologue<<<grid, 256, 0, s>>>(F->d_orig, (long)nslots, F->K,
-                                              d_b + (size_t)k * ldb, F->d_rec);
+    const int cap = nrhs > 1 ? rhs_per_launch(F) : 1;
+    RB_TRY(ensure_rhs(F, std::min(nrhs, cap), s));
+    for (int k0 = 0; k0 < nrhs; k0 += cap) {
+        const int nr = std::min(cap, nrhs - k0);
+        const int cb = F->epoch & 1, nb = cb ^ 1;
+        double *cur = F->d_xq + (size_t)cb * F->nr_alloc * nslots;
+        double *nxt = F->d_xq + (size_t)nb * F->nr_alloc * nslots;
+        // regions this launch polls must hold the sentinel
+        for (int r = 0; r < nr; r++)
+            if (!F->armed[cb][r]) {
+                k_fill_pending<<<(int)((nslots + 255) / 256), 256, 0, s>>>(
+                    cur + r * nslots, (long)nslots);
+                RB_HIP(hipGetLastError());
+            }
+        const double *b = d_b + (size_t)k0 * ldb;
+        const long tot = (long)nslots * nr;
+        const int grid = (int)((tot + 255) / 256);
+        if (nr == 1)
+            k_fused_prologue<<<grid, 256, 0, s>>>(F->d_orig, (long)nslots, F->K,
+                                                  b, F->d_rec);
+        else
+            k_fused_prologue_multi<<<grid, 256, 0, s>>>(F->d_orig, (long)nslots,
+                                                        nr, b, ldb, F->d_brec);
         RB_HIP(hipGetLastError());
         FusedArgs args{F->d_step0, F->d_gent0, F->d_pub0, F->d_rec,
                        F->d_gsrc, F->d_gval, F->d_gmeta, F->d_pub, cur, nxt,
-                       p->d_status, F->d_trace};
+                       F->d_brec, (long)nslots, F->W, p->d_status, F->d_trace};
+        void *kargs[] = {(void *)&args};
         mark_start(p, s);
-        RB_TRY(F->NDPP ? launch_ring<3>(F, args, s) : launch_ring<0>(F, args, s));
+        hipError_t e = hipLaunchCooperativeKernel(
+            nr == 1 ? pick_kernel<false>(F) : pick_kernel<true>(F),
+            dim3(F->W * nr), dim3(128), kargs, 0, s);
+        if (e != hipSuccess)
+            return status_of_hip(e, "hipLaunchCooperativeKernel(k_fused)");
         mark_end(p, s);
-        k_fused_epilogue<<<grid, 256, 0, s>>>(F->d_orig, (long)nslots, cur,
-                                              d_x + (size_t)k * ldx);
+        k_fused_epilogue<<<grid, 256, 0, s>>>(F->d_orig, (long)nslots, nr, cur,
+                                              d_x + (size_t)k0 * ldx, ldx);
         RB_HIP(hipGetLastError());
+        for (int r = 0; r < nr; r++) {
+            F->armed[cb][r] = 0;
+            F->armed[nb][r] = 1;
+        }
         F->epoch++;
     }
     return RB_OK;

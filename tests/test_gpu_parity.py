@@ -324,3 +324,46 @@ def test_kernel_timing_events(torch, schedule):
     p.set_timing(False)
     x2 = p.solve(b)
     assert np.array_equal(x.cpu().numpy(), x2.cpu().numpy())
+
+
+def _solve_many(torch, p, B, ld=None):
+    """Solve the columns of B (k, n) in one call; returns X (k, n)."""
+    import ctypes
+    from rbsptrsv._lib import check, lib
+    k, n = B.shape
+    ld = ld or n
+    Bp = np.zeros((k, ld))
+    Bp[:, :n] = B
+    Bd = torch.as_tensor(Bp, device="cuda")
+    Xd = torch.zeros((k, ld), dtype=torch.float64, device="cuda")
+    check(lib().rb_solve(p._h, ctypes.c_void_p(Bd.data_ptr()), ld,
+                         ctypes.c_void_p(Xd.data_ptr()), ld, k, None))
+    torch.cuda.synchronize()
+    return Xd.cpu().numpy()[:, :n]
+
+
+@pytest.mark.parametrize("name,make", [
+    ("laplacian60x50", lambda: gen.laplacian(60, 50)),
+    ("random2000", lambda: gen.random_lower(2000, 6, seed=2)),
+    ("banded20000-upper", lambda: gen.banded(20000, seed=5).transpose()),
+])
+def test_fused_many_rhs_match_single(oracle, torch, name, make):
+    """Several right-hand sides per fused launch (waves replicated per
+    column) give bitwise the single-column results; counts that need more
+    than one launch, and changing counts between calls, included."""
+    f = make()
+    p = _plan(f, 2, schedule=2)
+    rng = np.random.default_rng(7)
+    re, part, bnz, an = oracle.full_pipeline(f.colptr, f.rowidx, f.val, f.n,
+                                             f.substitution, 2)
+    for k in (3, 1, 70, 2, 131, 5):
+        _say("fused rhs", name, k)
+        B = rng.uniform(-1, 1, (k, f.n))
+        X = _solve_many(torch, p, B, ld=f.n + 3)
+        p.check()
+        for j in sorted({0, k // 2, k - 1}):
+            xs = p.solve(torch.as_tensor(B[j], device="cuda")).cpu().numpy()
+            assert np.array_equal(X[j].view(np.int64), xs.view(np.int64)), (k, j)
+            xo = oracle.solve_blocks(re, f.n, 2, f.substitution, part, B[j])
+            assert rel_err(X[j], xo) < X_TOL
+        assert max(residual(f, X[j], B[j]) for j in range(k)) < RES_TOL
