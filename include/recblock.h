@@ -35,7 +35,8 @@ typedef enum {
     RB_ERR_ZERO_DIAG = 5,     /* a zero diagonal value */
     RB_ERR_HIP = 6,           /* a HIP runtime call failed */
     RB_ERR_NO_MEMORY = 7,     /* device allocation failed */
-    RB_ERR_INTERNAL = 8
+    RB_ERR_INTERNAL = 8,
+    RB_ERR_NOT_CONVERGED = 9   /* FP32 refinement missed its tolerance */
 } rb_status;
 
 #define RB_SUBSTITUTION_FORWARD 0
@@ -53,7 +54,10 @@ typedef struct {
     int fused_waves;   /* fused schedule: cap on consumer waves (0 = auto) */
     int no_graph;      /* blocked schedule: 1 = launch the block walk kernel
                           by kernel instead of replaying a captured graph */
-    int reserved[4];
+    int mixed;         /* 1 = also keep FP32 copies of the block values and
+                          the FP64 matrix for residuals: enables
+                          rb_solve_f32ir */
+    int reserved[3];
 } rb_opts;
 
 typedef struct rb_plan rb_plan;
@@ -186,6 +190,22 @@ rb_status rb_plan_block_csr(const rb_plan *plan, int i, int *rowptr,
  * whole solves only). */
 rb_status rb_plan_trace(const rb_plan *plan, unsigned long long *counters,
                         int *nwaves);
+
+/* FP32 solves with FP64 iterative refinement (plan created with
+ * rb_opts.mixed = 1).  Per right-hand side: x = 0, r = b; repeat
+ * { d = solve((float) r) with the FP32 copies of the block values, in the
+ * blocked schedule's kernels; x += d in FP64; r = b - A x in FP64 on the
+ * device } until max_i |r_i| <= tol * max_i |b_i| or max_iter corrections.
+ * FP64 is used only for the residual and the update of x.  d_b / d_x as in
+ * rb_solve.  *iters receives the most corrections any column needed and
+ * *relres the largest final max|r| / max|b| (either may be NULL).  The call
+ * synchronises on stream once per correction (to test convergence).
+ * Returns RB_ERR_NOT_CONVERGED when a column missed tol (x then holds the
+ * last iterate).  No counterpart in the reference (FP64 only). */
+rb_status rb_solve_f32ir(rb_plan *plan, const double *d_b, int ldb,
+                         double *d_x, int ldx, int nrhs, double tol,
+                         int max_iter, void *stream, int *iters,
+                         double *relres);
 
 /* Kernel timing for measurement harnesses.  While enabled, every solve
  * records HIP events on its stream around its main launch: the fused

@@ -166,6 +166,7 @@ rb_status rb_plan_create(const int *d_colptr, const int *d_rowidx,
     p->want_schedule = o.schedule;
     p->fused_waves = o.fused_waves;
     p->graph_failed = o.no_graph != 0;  // eager launches only
+    p->mixed = o.mixed != 0;
     rb_status st = build_plan(p, d_colptr, d_rowidx, d_val,
                               (hipStream_t)stream);
     if (st == RB_OK) st = choose_schedule(p, (hipStream_t)stream);
@@ -212,6 +213,29 @@ rb_status rb_plan_check(rb_plan *p, void *stream) {
                   "arrived (x holds NaN for the affected rows)");
         return RB_ERR_INTERNAL;
     }
+    return ok();
+}
+
+rb_status rb_solve_f32ir(rb_plan *p, const double *d_b, int ldb, double *d_x,
+                         int ldx, int nrhs, double tol, int max_iter,
+                         void *stream, int *iters, double *relres) {
+    if (!p || nrhs < 0 || (nrhs > 0 && (!d_b || !d_x)) ||
+        (nrhs > 0 && (ldb < p->m || ldx < p->m)) || max_iter < 1 ||
+        !(tol > 0.0)) {
+        set_error(RB_ERR_INVALID_ARG, "invalid refinement arguments");
+        return RB_ERR_INVALID_ARG;
+    }
+    if (!p->mixed) {
+        set_error(RB_ERR_INVALID_ARG,
+                  "plan was created without rb_opts.mixed (no FP32 copies)");
+        return RB_ERR_INVALID_ARG;
+    }
+    if (iters) *iters = 0;
+    if (relres) *relres = 0.0;
+    if (nrhs == 0) return ok();
+    rb_status st = solve_f32ir(p, d_b, ldb, d_x, ldx, nrhs, tol, max_iter,
+                               (hipStream_t)stream, iters, relres);
+    if (st != RB_OK) return st;
     return ok();
 }
 

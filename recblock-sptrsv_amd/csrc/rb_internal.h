@@ -31,6 +31,7 @@ rb_status status_of_hip(hipError_t e, const char *what);
 // Sentinel bit pattern of an x slot that is not solved yet: a signalling
 // NaN payload that IEEE arithmetic never produces (results are quiet NaNs).
 constexpr uint64_t kPending = 0xFFF4A5A55A5A0001ull;
+constexpr uint32_t kPending32 = 0xFFA5A001u;  // the same for FP32 slots
 constexpr int kLongRow = 2048;  // LONGROW_THRESHOLD, R/utils_spmv_cuda.h:9
 
 // Device allocation helper that tracks bytes held by a plan.
@@ -148,6 +149,17 @@ struct rb_plan {
     // kernel timing (rb_plan_set_timing)
     bool timing = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    // FP32 solve + FP64 refinement (rb_opts.mixed)
+    bool mixed = false;
+    float *d_val32 = nullptr, *d_invd32 = nullptr;   // slot values in FP32
+    float *d_bperm32 = nullptr, *d_xperm32 = nullptr;
+    void *graph_exec32 = nullptr;                    // FP32 block walk
+    int *d_rptr = nullptr, *d_rcol = nullptr;        // FP64 matrix for the
+    double *d_rval = nullptr;                        // residual (see below)
+    double *d_r = nullptr;                           // residual, FP64
+    float *d_r32 = nullptr, *d_d32 = nullptr;        // residual / correction
+    unsigned long long *d_norms = nullptr;           // max |b|, max |r|
 };
 
 namespace rb {
@@ -155,6 +167,9 @@ rb_status build_plan(rb_plan *p, const int *d_colptr, const int *d_rowidx,
                      const double *d_val, hipStream_t s);
 rb_status solve_blocked(rb_plan *p, const double *d_b, int ldb, double *d_x,
                         int ldx, int nrhs, hipStream_t s);
+rb_status solve_f32ir(rb_plan *p, const double *d_b, int ldb, double *d_x,
+                      int ldx, int nrhs, double tol, int max_iter,
+                      hipStream_t s, int *iters, double *relres);
 // fused schedule (rb_fused.hip)
 rb_status build_fused(rb_plan *p, const int *d_prptr, const int *d_ccol,
                       const double *d_cval, int Wmax, hipStream_t s);

@@ -139,6 +139,18 @@ __global__ void k_pcolcopy(const int *__restrict__ colptr,
     }
 }
 
+// column ids of the level-ordered CSR mapped back to original ids
+__global__ void k_orig_cols(const int *__restrict__ ccol,
+                            const int *__restrict__ lf, int nnz, int *rcol) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nnz) rcol[k] = lf[ccol[k]];
+}
+
+__global__ void k_f64_to_f32(const double *__restrict__ a, long n, float *o) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) o[i] = (float)a[i];
+}
+
 __global__ void k_iota(int *a, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = i;
@@ -816,7 +828,36 @@ rb_status build_plan(rb_plan *p, const int *d_colptr, const int *d_rowidx,
     if (p->want_schedule != RB_SCHED_BLOCKED && m > 0)
         RB_TRY(build_fused(p, prptr, ccol, cval, p->fused_waves, s));
 
-    // ---- 10. solve scratch
+    // ---- 10. FP32 + FP64-refinement storage: FP32 copies of the slot
+    // values, and the level-ordered CSR with original ids (row i is
+    // original row levelItem[i]) for FP64 residuals in the original order
+    if (p->mixed && m > 0) {
+        RB_TRY(A.alloc(&p->d_val32, offnnz));
+        RB_TRY(A.alloc(&p->d_invd32, m));
+        if (offnnz > 0)
+            k_f64_to_f32<<<nblk(offnnz), kT, 0, s>>>(p->d_val, offnnz, p->d_val32);
+        k_f64_to_f32<<<nblk(m), kT, 0, s>>>(p->d_invd, m, p->d_invd32);
+        RB_TRY(A.alloc(&p->d_rptr, m + 1));
+        RB_TRY(A.alloc(&p->d_rcol, nnz));
+        RB_TRY(A.alloc(&p->d_rval, nnz));
+        RB_HIP(hipMemcpyAsync(p->d_rptr, prptr, sizeof(int) * (size_t)(m + 1),
+                              hipMemcpyDeviceToDevice, s));
+        if (nnz > 0) {
+            k_orig_cols<<<nblk(nnz), kT, 0, s>>>(ccol, p->d_levelItem, nnz,
+                                                 p->d_rcol);
+            RB_HIP(hipMemcpyAsync(p->d_rval, cval, sizeof(double) * (size_t)nnz,
+                                  hipMemcpyDeviceToDevice, s));
+        }
+        RB_HIP(hipGetLastError());
+        RB_TRY(A.alloc(&p->d_bperm32, m));
+        RB_TRY(A.alloc(&p->d_xperm32, m));
+        RB_TRY(A.alloc(&p->d_r, m));
+        RB_TRY(A.alloc(&p->d_r32, m));
+        RB_TRY(A.alloc(&p->d_d32, m));
+        RB_TRY(A.alloc(&p->d_norms, 2));
+    }
+
+    // ---- 11. solve scratch
     RB_TRY(A.alloc(&p->d_bperm, m));
     RB_TRY(A.alloc(&p->d_xperm, m));
     RB_TRY(A.alloc(&p->d_tickets, ntri));

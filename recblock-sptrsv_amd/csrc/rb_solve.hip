@@ -18,6 +18,10 @@
 //   non-empty row (R/utils_spmv_cuda.h:60-203), plus a workgroup per row
 //   longer than 2048 entries (the long-row pass :30-58).
 // Every sum runs in a fixed order, so a solve is bitwise reproducible.
+//
+// The kernels are templates on the value type: the FP64 walk is the plain
+// solve; the FP32 walk (FP32 copies of the slot values) is the inner solver
+// of the FP32 + FP64 iterative refinement at the end of this file.
 #include "rb_internal.h"
 
 namespace rb {
@@ -27,91 +31,123 @@ namespace {
 constexpr int kT = 256;
 constexpr unsigned kSpinLimit = 1u << 22;  // about seconds of polling
 
-__device__ inline double ld_x(const double *p) {
-    return __longlong_as_double((long long)__hip_atomic_load(
-        (const unsigned long long *)p, __ATOMIC_RELAXED,
-        __HIP_MEMORY_SCOPE_AGENT));
+// value type <-> same-width unsigned bits, and the type's sentinel
+template <class T> struct Bits;
+template <> struct Bits<double> {
+    using U = unsigned long long;
+    static constexpr U kPend = kPending;
+    static constexpr U kQNaN = 0x7FF8000000000000ull;
+};
+template <> struct Bits<float> {
+    using U = unsigned int;
+    static constexpr U kPend = kPending32;
+    static constexpr U kQNaN = 0x7FC00000u;
+};
+
+template <class T>
+__device__ inline T ld_x(const T *p) {
+    using U = typename Bits<T>::U;
+    const U u = __hip_atomic_load((const U *)p, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+    return __builtin_bit_cast(T, u);
 }
 
-__device__ inline void st_x(double *p, double v) {
-    __hip_atomic_store((unsigned long long *)p,
-                       (unsigned long long)__double_as_longlong(v),
-                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+template <class T>
+__device__ inline void st_x(T *p, T v) {
+    using U = typename Bits<T>::U;
+    __hip_atomic_store((U *)p, __builtin_bit_cast(U, v), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__global__ void k_prologue(const double *__restrict__ b,
-                           const int *__restrict__ lf, int m, double *bperm,
-                           double *xperm, int *tickets, int ntri) {
+template <class T>
+__device__ inline bool is_pending(T v) {
+    return __builtin_bit_cast(typename Bits<T>::U, v) == Bits<T>::kPend;
+}
+
+template <class T>
+__global__ void k_prologue(const T *__restrict__ b, const int *__restrict__ lf,
+                           int m, T *bperm, T *xperm, int *tickets, int ntri) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < m) {
         bperm[i] = b[lf[i]];
-        ((unsigned long long *)xperm)[i] = kPending;
+        ((typename Bits<T>::U *)xperm)[i] = Bits<T>::kPend;
     }
     if (i < ntri) tickets[i] = 0;
 }
 
-__global__ void k_epilogue(const double *__restrict__ xperm,
-                           const int *__restrict__ lf, int m, double *x) {
+template <class T>
+__global__ void k_epilogue(const T *__restrict__ xperm,
+                           const int *__restrict__ lf, int m, T *x) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < m) x[lf[i]] = xperm[i];
 }
 
-__global__ void k_tri_diag(const double *__restrict__ invd,
-                           const double *__restrict__ bperm, int up, int n,
-                           double *xperm) {
+template <class T>
+__global__ void k_tri_diag(const T *__restrict__ invd,
+                           const T *__restrict__ bperm, int up, int n,
+                           T *xperm) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) st_x(&xperm[up + i], bperm[up + i] * invd[up + i]);
 }
 
 // level-set, thread per row; rows of one local level
+template <class T>
 __global__ void k_tri_level_thread(const int *__restrict__ rowptr,
                                    const int *__restrict__ col,
-                                   const double *__restrict__ val,
-                                   const double *__restrict__ invd,
-                                   const double *__restrict__ bperm,
+                                   const T *__restrict__ val,
+                                   const T *__restrict__ invd,
+                                   const T *__restrict__ bperm,
                                    const int *__restrict__ rows, int nrows,
-                                   int sb, int up, double *xperm) {
+                                   int sb, int up, T *xperm) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nrows) return;
     const int r = rows[k];
     const int s = rowptr[sb + r], e = rowptr[sb + r + 1];
-    double acc = 0.0;
+    T acc = 0;
     for (int j = s; j < e; j++) acc += val[j] * xperm[up + col[j]];
     st_x(&xperm[up + r], (bperm[up + r] - acc) * invd[up + r]);
 }
 
-__device__ inline double wave_sum(double v) {
+template <class T>
+__device__ inline T wave_sum(T v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
 
+__device__ inline double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
 // level-set, wave per row
+template <class T>
 __global__ void k_tri_level_wave(const int *__restrict__ rowptr,
                                  const int *__restrict__ col,
-                                 const double *__restrict__ val,
-                                 const double *__restrict__ invd,
-                                 const double *__restrict__ bperm,
+                                 const T *__restrict__ val,
+                                 const T *__restrict__ invd,
+                                 const T *__restrict__ bperm,
                                  const int *__restrict__ rows, int nrows,
-                                 int sb, int up, double *xperm) {
+                                 int sb, int up, T *xperm) {
     const int k = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     if (k >= nrows) return;
     const int r = rows[k];
     const int s = rowptr[sb + r], e = rowptr[sb + r + 1];
-    double acc = 0.0;
+    T acc = 0;
     for (int j = s + lane; j < e; j += 64) acc += val[j] * xperm[up + col[j]];
     acc = wave_sum(acc);
     if (lane == 0) st_x(&xperm[up + r], (bperm[up + r] - acc) * invd[up + r]);
 }
 
 // sync-free CSR pull, lane per row, 64 consecutive rows per wave ticket
+template <class T>
 __global__ void __launch_bounds__(kT)
 k_tri_syncfree(const int *__restrict__ rowptr, const int *__restrict__ col,
-               const double *__restrict__ val,
-               const double *__restrict__ invd,
-               const double *__restrict__ bperm, int sb, int up, int bm,
-               int subst, int *ticket, double *xperm, int *status) {
+               const T *__restrict__ val, const T *__restrict__ invd,
+               const T *__restrict__ bperm, int sb, int up, int bm,
+               int subst, int *ticket, T *xperm, int *status) {
     const int lane = threadIdx.x & 63;
     int tile = 0;
     if (lane == 0) tile = atomicAdd(ticket, 1);
@@ -120,22 +156,22 @@ k_tri_syncfree(const int *__restrict__ rowptr, const int *__restrict__ col,
     const bool valid = pos < bm;
     const int r = subst == 0 ? pos : bm - 1 - pos;
     int j = 0, e = 0;
-    double bv = 0.0, iv = 0.0;
+    T bv = 0, iv = 0;
     if (valid) {
         j = rowptr[sb + r];
         e = rowptr[sb + r + 1];
         bv = bperm[up + r];
         iv = invd[up + r];
     }
-    double acc = 0.0;
+    T acc = 0;
     bool pend = valid;
-    const double *xb = xperm + up;
+    const T *xb = xperm + up;
     unsigned spins = 0;
     while (__any(pend)) {
         if (pend) {
             while (j < e) {
-                const double xv = ld_x(&xb[col[j]]);
-                if (__double_as_longlong(xv) == (long long)kPending) break;
+                const T xv = ld_x(&xb[col[j]]);
+                if (is_pending(xv)) break;
                 acc += val[j] * xv;
                 j++;
             }
@@ -150,7 +186,7 @@ k_tri_syncfree(const int *__restrict__ rowptr, const int *__restrict__ col,
             if (++spins > kSpinLimit) {
                 if (pend) {
                     atomicOr(status, 1);
-                    st_x(&xperm[up + r], __longlong_as_double(0x7FF8000000000000ll));
+                    st_x(&xperm[up + r], __builtin_bit_cast(T, Bits<T>::kQNaN));
                     pend = false;
                 }
             } else {
@@ -161,87 +197,91 @@ k_tri_syncfree(const int *__restrict__ rowptr, const int *__restrict__ col,
 }
 
 // square block SpMV, CSR, thread per row; b[up + r] -= A[r,:] x[left:]
+template <class T>
 __global__ void k_sq_csr_thread(const int *__restrict__ rowptr,
                                 const int *__restrict__ col,
-                                const double *__restrict__ val,
-                                const double *__restrict__ xperm, int sb,
-                                int up, int left, int bm, double *bperm) {
+                                const T *__restrict__ val,
+                                const T *__restrict__ xperm, int sb,
+                                int up, int left, int bm, T *bperm) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= bm) return;
     const int s = rowptr[sb + r], e = rowptr[sb + r + 1];
     if (s == e || e - s > kLongRow) return;
-    double acc = 0.0;
+    T acc = 0;
     for (int j = s; j < e; j++) acc += val[j] * xperm[left + col[j]];
     bperm[up + r] -= acc;
 }
 
+template <class T>
 __global__ void k_sq_csr_wave(const int *__restrict__ rowptr,
                               const int *__restrict__ col,
-                              const double *__restrict__ val,
-                              const double *__restrict__ xperm, int sb,
-                              int up, int left, int bm, double *bperm) {
+                              const T *__restrict__ val,
+                              const T *__restrict__ xperm, int sb,
+                              int up, int left, int bm, T *bperm) {
     const int r = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     if (r >= bm) return;
     const int s = rowptr[sb + r], e = rowptr[sb + r + 1];
     if (s == e || e - s > kLongRow) return;
-    double acc = 0.0;
+    T acc = 0;
     for (int j = s + lane; j < e; j += 64) acc += val[j] * xperm[left + col[j]];
     acc = wave_sum(acc);
     if (lane == 0) bperm[up + r] -= acc;
 }
 
+template <class T>
 __global__ void k_sq_dcsr_thread(const int *__restrict__ rowptr,
                                  const int *__restrict__ col,
-                                 const double *__restrict__ val,
-                                 const double *__restrict__ xperm,
+                                 const T *__restrict__ val,
+                                 const T *__restrict__ xperm,
                                  const int *__restrict__ rows, int nrows,
-                                 int sb, int up, int left, double *bperm) {
+                                 int sb, int up, int left, T *bperm) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nrows) return;
     const int r = rows[k];
     const int s = rowptr[sb + r], e = rowptr[sb + r + 1];
     if (e - s > kLongRow) return;
-    double acc = 0.0;
+    T acc = 0;
     for (int j = s; j < e; j++) acc += val[j] * xperm[left + col[j]];
     bperm[up + r] -= acc;
 }
 
+template <class T>
 __global__ void k_sq_dcsr_wave(const int *__restrict__ rowptr,
                                const int *__restrict__ col,
-                               const double *__restrict__ val,
-                               const double *__restrict__ xperm,
+                               const T *__restrict__ val,
+                               const T *__restrict__ xperm,
                                const int *__restrict__ rows, int nrows,
-                               int sb, int up, int left, double *bperm) {
+                               int sb, int up, int left, T *bperm) {
     const int k = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     if (k >= nrows) return;
     const int r = rows[k];
     const int s = rowptr[sb + r], e = rowptr[sb + r + 1];
     if (e - s > kLongRow) return;
-    double acc = 0.0;
+    T acc = 0;
     for (int j = s + lane; j < e; j += 64) acc += val[j] * xperm[left + col[j]];
     acc = wave_sum(acc);
     if (lane == 0) bperm[up + r] -= acc;
 }
 
 // one workgroup per long row (> 2048 entries), fixed-order tree reduction
+template <class T>
 __global__ void __launch_bounds__(kT)
 k_sq_long(const int *__restrict__ rowptr, const int *__restrict__ col,
-          const double *__restrict__ val, const double *__restrict__ xperm,
-          const int *__restrict__ rows, int sb, int up, int left,
-          double *bperm) {
+          const T *__restrict__ val, const T *__restrict__ xperm,
+          const int *__restrict__ rows, int sb, int up, int left, T *bperm) {
     const int r = rows[blockIdx.x];
     const int s = rowptr[sb + r], e = rowptr[sb + r + 1];
-    double acc = 0.0;
+    T acc = 0;
     for (int j = s + threadIdx.x; j < e; j += kT)
         acc += val[j] * xperm[left + col[j]];
     acc = wave_sum(acc);
-    __shared__ double sh[kT / 64];
+    __shared__ T sh[kT / 64];
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
     __syncthreads();
     if (threadIdx.x == 0) {
-        double t = 0.0;
+        T t = 0;
         for (int w = 0; w < kT / 64; w++) t += sh[w];
         bperm[up + r] -= t;
     }
@@ -249,92 +289,144 @@ k_sq_long(const int *__restrict__ rowptr, const int *__restrict__ col,
 
 inline int nblk(long n, int t = kT) { return (int)((n + t - 1) / t); }
 
-}  // namespace
-
-// The block walk of one solve: kernels on plan-owned buffers only, so it
-// can be captured once into a graph and replayed.
-static void launch_blocks(rb_plan *p, hipStream_t s) {
-        for (int i = 0; i < p->nblocks; i++) {
-            const Block &B = p->blocks[i];
-            if (B.blk_m == 0) continue;
-            if (B.kind == 0) {
-                switch (B.method) {
-                case 0:
-                    k_tri_diag<<<nblk(B.blk_m), kT, 0, s>>>(
-                        p->d_invd, p->d_bperm, B.up, B.blk_m, p->d_xperm);
-                    break;
-                case 2:
-                    for (int L = 0; L < B.nlv; L++) {
-                        const int a = B.lvptr_h[L], c = B.lvptr_h[L + 1];
-                        const int nr = c - a;
-                        if (nr <= 0) continue;
-                        const int *rows = p->d_lvitem + B.lv_base + a;
-                        if (L == 0 || B.nnzlv_h[L] / nr <= 15)
-                            k_tri_level_thread<<<nblk(nr), kT, 0, s>>>(
-                                p->d_rowptr, p->d_col, p->d_val, p->d_invd,
-                                p->d_bperm, rows, nr, B.slot_base, B.up,
-                                p->d_xperm);
-                        else
-                            k_tri_level_wave<<<nblk((long)nr * 64), kT, 0, s>>>(
-                                p->d_rowptr, p->d_col, p->d_val, p->d_invd,
-                                p->d_bperm, rows, nr, B.slot_base, B.up,
-                                p->d_xperm);
-                    }
-                    break;
-                default:  // 1 and 3
-                    k_tri_syncfree<<<nblk((long)B.blk_m, 64), 64, 0, s>>>(
-                        p->d_rowptr, p->d_col, p->d_val, p->d_invd,
-                        p->d_bperm, B.slot_base, B.up, B.blk_m, p->subst,
-                        p->d_tickets + i / 2, p->d_xperm, p->d_status);
-                    break;
-                }
-            } else {
-                const int *rows = p->d_dcsr_rows + B.dcsr_base;
-                switch (B.method) {
-                case -1:
-                    break;
-                case 0:
-                    k_sq_csr_thread<<<nblk(B.blk_m), kT, 0, s>>>(
-                        p->d_rowptr, p->d_col, p->d_val, p->d_xperm,
-                        B.slot_base, B.up, B.left, B.blk_m, p->d_bperm);
-                    break;
-                case 2:
-                    k_sq_csr_wave<<<nblk((long)B.blk_m * 64), kT, 0, s>>>(
-                        p->d_rowptr, p->d_col, p->d_val, p->d_xperm,
-                        B.slot_base, B.up, B.left, B.blk_m, p->d_bperm);
-                    break;
-                case 1:
-                    k_sq_dcsr_thread<<<nblk(B.m_new), kT, 0, s>>>(
-                        p->d_rowptr, p->d_col, p->d_val, p->d_xperm, rows,
-                        B.m_new, B.slot_base, B.up, B.left, p->d_bperm);
-                    break;
-                default:
-                    k_sq_dcsr_wave<<<nblk((long)B.m_new * 64), kT, 0, s>>>(
-                        p->d_rowptr, p->d_col, p->d_val, p->d_xperm, rows,
-                        B.m_new, B.slot_base, B.up, B.left, p->d_bperm);
-                    break;
-                }
-                if (B.method >= 0 && B.longrow > 0)
-                    k_sq_long<<<B.longrow, kT, 0, s>>>(
-                        p->d_rowptr, p->d_col, p->d_val, p->d_xperm,
-                        p->d_long_rows + B.long_base, B.slot_base, B.up,
-                        B.left, p->d_bperm);
-            }
-        }
+// ---------------------------------------------------- refinement kernels
+__global__ void k_to_f32(const double *__restrict__ a, int m, float *o) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) o[i] = (float)a[i];
 }
 
-// Capture the block walk into a graph (once per plan).  Returns false when
-// capture is not possible; the walk is then launched kernel by kernel.
-static bool capture_blocks(rb_plan *p) {
+// x += d (FP64 update with the FP32 correction)
+__global__ void k_add_f32(const float *__restrict__ d, int m, double *x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) x[i] += (double)d[i];
+}
+
+// max |v| over the vector, as the bits of a non-negative double (ordered
+// like the values) into *out (cleared by the caller)
+__global__ void k_absmax(const double *__restrict__ v, int m,
+                         unsigned long long *out) {
+    double mx = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m;
+         i += gridDim.x * blockDim.x)
+        mx = fmax(mx, fabs(v[i]));
+    mx = wave_max(mx);
+    if ((threadIdx.x & 63) == 0)
+        atomicMax(out, (unsigned long long)__double_as_longlong(mx));
+}
+
+// r = b - A x in the original ordering, FP64: row i of the level-ordered
+// CSR is original row lf[i], its columns hold original ids; also max |r|
+__global__ void k_residual(const int *__restrict__ rptr,
+                           const int *__restrict__ rcol,
+                           const double *__restrict__ rval,
+                           const int *__restrict__ lf, int m,
+                           const double *__restrict__ b,
+                           const double *__restrict__ x, double *r,
+                           unsigned long long *rmax) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double a = 0.0;
+    if (i < m) {
+        const int o = lf[i];
+        double acc = 0.0;
+        for (int k = rptr[i]; k < rptr[i + 1]; k++) acc += rval[k] * x[rcol[k]];
+        const double v = b[o] - acc;
+        r[o] = v;
+        a = fabs(v);
+    }
+    a = wave_max(a);
+    if ((threadIdx.x & 63) == 0)
+        atomicMax(rmax, (unsigned long long)__double_as_longlong(a));
+}
+
+}  // namespace
+
+// The block walk of one solve in value type T: kernels on plan-owned
+// buffers only, so it can be captured once into a graph and replayed.
+template <class T>
+static void launch_blocks(rb_plan *p, const T *val, const T *invd, T *bperm,
+                          T *xperm, hipStream_t s) {
+    for (int i = 0; i < p->nblocks; i++) {
+        const Block &B = p->blocks[i];
+        if (B.blk_m == 0) continue;
+        if (B.kind == 0) {
+            switch (B.method) {
+            case 0:
+                k_tri_diag<T><<<nblk(B.blk_m), kT, 0, s>>>(invd, bperm, B.up,
+                                                           B.blk_m, xperm);
+                break;
+            case 2:
+                for (int L = 0; L < B.nlv; L++) {
+                    const int a = B.lvptr_h[L], c = B.lvptr_h[L + 1];
+                    const int nr = c - a;
+                    if (nr <= 0) continue;
+                    const int *rows = p->d_lvitem + B.lv_base + a;
+                    if (L == 0 || B.nnzlv_h[L] / nr <= 15)
+                        k_tri_level_thread<T><<<nblk(nr), kT, 0, s>>>(
+                            p->d_rowptr, p->d_col, val, invd, bperm, rows, nr,
+                            B.slot_base, B.up, xperm);
+                    else
+                        k_tri_level_wave<T><<<nblk((long)nr * 64), kT, 0, s>>>(
+                            p->d_rowptr, p->d_col, val, invd, bperm, rows, nr,
+                            B.slot_base, B.up, xperm);
+                }
+                break;
+            default:  // 1 and 3
+                k_tri_syncfree<T><<<nblk((long)B.blk_m, 64), 64, 0, s>>>(
+                    p->d_rowptr, p->d_col, val, invd, bperm, B.slot_base,
+                    B.up, B.blk_m, p->subst, p->d_tickets + i / 2, xperm,
+                    p->d_status);
+                break;
+            }
+        } else {
+            const int *rows = p->d_dcsr_rows + B.dcsr_base;
+            switch (B.method) {
+            case -1:
+                break;
+            case 0:
+                k_sq_csr_thread<T><<<nblk(B.blk_m), kT, 0, s>>>(
+                    p->d_rowptr, p->d_col, val, xperm, B.slot_base, B.up,
+                    B.left, B.blk_m, bperm);
+                break;
+            case 2:
+                k_sq_csr_wave<T><<<nblk((long)B.blk_m * 64), kT, 0, s>>>(
+                    p->d_rowptr, p->d_col, val, xperm, B.slot_base, B.up,
+                    B.left, B.blk_m, bperm);
+                break;
+            case 1:
+                k_sq_dcsr_thread<T><<<nblk(B.m_new), kT, 0, s>>>(
+                    p->d_rowptr, p->d_col, val, xperm, rows, B.m_new,
+                    B.slot_base, B.up, B.left, bperm);
+                break;
+            default:
+                k_sq_dcsr_wave<T><<<nblk((long)B.m_new * 64), kT, 0, s>>>(
+                    p->d_rowptr, p->d_col, val, xperm, rows, B.m_new,
+                    B.slot_base, B.up, B.left, bperm);
+                break;
+            }
+            if (B.method >= 0 && B.longrow > 0)
+                k_sq_long<T><<<B.longrow, kT, 0, s>>>(
+                    p->d_rowptr, p->d_col, val, xperm,
+                    p->d_long_rows + B.long_base, B.slot_base, B.up, B.left,
+                    bperm);
+        }
+    }
+}
+
+// Capture a block walk into a graph (once per plan and value type).
+// Returns false when capture is not possible; the walk is then launched
+// kernel by kernel.
+template <class T>
+static bool capture_blocks(rb_plan *p, void **exec, const T *val,
+                           const T *invd, T *bperm, T *xperm) {
     if (p->graph_failed) return false;
-    if (p->graph_exec) return true;
+    if (*exec) return true;
     hipStream_t cs = nullptr;
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
     bool ok = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) == hipSuccess;
     if (ok) ok = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) == hipSuccess;
     if (ok) {
-        launch_blocks(p, cs);
+        launch_blocks<T>(p, val, invd, bperm, xperm, cs);
         ok = hipStreamEndCapture(cs, &g) == hipSuccess && g != nullptr;
     }
     if (ok) ok = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) == hipSuccess;
@@ -345,34 +437,100 @@ static bool capture_blocks(rb_plan *p) {
         p->graph_failed = true;
         return false;
     }
-    p->graph_exec = ge;
+    *exec = ge;
     return true;
 }
 
 void destroy_graph(rb_plan *p) {
     if (p->graph_exec) (void)hipGraphExecDestroy((hipGraphExec_t)p->graph_exec);
+    if (p->graph_exec32) (void)hipGraphExecDestroy((hipGraphExec_t)p->graph_exec32);
     p->graph_exec = nullptr;
+    p->graph_exec32 = nullptr;
+}
+
+template <class T>
+static rb_status walk(rb_plan *p, void **exec, const T *val, const T *invd,
+                      T *bperm, T *xperm, const T *b, T *x, hipStream_t s) {
+    const int m = p->m;
+    const bool graph = capture_blocks<T>(p, exec, val, invd, bperm, xperm);
+    const int pro = nblk(m > p->ntri ? m : p->ntri);
+    k_prologue<T><<<pro, kT, 0, s>>>(b, p->d_levelItem, m, bperm, xperm,
+                                     p->d_tickets, p->ntri);
+    mark_start(p, s);
+    if (graph)
+        RB_HIP(hipGraphLaunch((hipGraphExec_t)*exec, s));
+    else
+        launch_blocks<T>(p, val, invd, bperm, xperm, s);
+    mark_end(p, s);
+    k_epilogue<T><<<nblk(m), kT, 0, s>>>(xperm, p->d_levelItem, m, x);
+    RB_HIP(hipGetLastError());
+    return RB_OK;
 }
 
 rb_status solve_blocked(rb_plan *p, const double *d_b, int ldb, double *d_x,
                         int ldx, int nrhs, hipStream_t s) {
+    if (p->m == 0) return RB_OK;
+    for (int k = 0; k < nrhs; k++)
+        RB_TRY(walk<double>(p, &p->graph_exec, p->d_val, p->d_invd,
+                            p->d_bperm, p->d_xperm, d_b + (size_t)k * ldb,
+                            d_x + (size_t)k * ldx, s));
+    return RB_OK;
+}
+
+// FP32 solves refined in FP64 (rb_solve_f32ir): per right-hand side,
+// x = 0, r = b; repeat { d = solve32((float) r); x += d; r = b - A x (FP64) }
+// until max|r| <= tol * max|b| or max_iter corrections.
+rb_status solve_f32ir(rb_plan *p, const double *d_b, int ldb, double *d_x,
+                      int ldx, int nrhs, double tol, int max_iter,
+                      hipStream_t s, int *iters, double *relres) {
     const int m = p->m;
-    if (m == 0) return RB_OK;
-    const bool graph = capture_blocks(p);
-    for (int k = 0; k < nrhs; k++) {
+    int it_max = 0;
+    double rel_max = 0.0;
+    bool bad = false;
+    for (int k = 0; k < nrhs && m > 0; k++) {
         const double *b = d_b + (size_t)k * ldb;
         double *x = d_x + (size_t)k * ldx;
-        const int pro = nblk(m > p->ntri ? m : p->ntri);
-        k_prologue<<<pro, kT, 0, s>>>(b, p->d_levelItem, m, p->d_bperm,
-                                      p->d_xperm, p->d_tickets, p->ntri);
-        mark_start(p, s);
-        if (graph)
-            RB_HIP(hipGraphLaunch((hipGraphExec_t)p->graph_exec, s));
-        else
-            launch_blocks(p, s);
-        mark_end(p, s);
-        k_epilogue<<<nblk(m), kT, 0, s>>>(p->d_xperm, p->d_levelItem, m, x);
-        RB_HIP(hipGetLastError());
+        unsigned long long *nrm = p->d_norms;
+        RB_HIP(hipMemsetAsync(nrm, 0, 2 * sizeof(unsigned long long), s));
+        k_absmax<<<std::min(nblk(m), 1024), kT, 0, s>>>(b, m, nrm);
+        RB_HIP(hipMemsetAsync(x, 0, sizeof(double) * (size_t)m, s));
+        RB_HIP(hipMemcpyAsync(p->d_r, b, sizeof(double) * (size_t)m,
+                              hipMemcpyDeviceToDevice, s));
+        unsigned long long hb = 0;
+        RB_HIP(hipMemcpyAsync(&hb, nrm, sizeof(hb), hipMemcpyDeviceToHost, s));
+        RB_HIP(hipStreamSynchronize(s));
+        const double bn = __builtin_bit_cast(double, hb);
+        double rel = 0.0;
+        int it = 0;
+        while (bn > 0.0 && it < max_iter) {
+            it++;
+            k_to_f32<<<nblk(m), kT, 0, s>>>(p->d_r, m, p->d_r32);
+            RB_TRY(walk<float>(p, &p->graph_exec32, p->d_val32, p->d_invd32,
+                               p->d_bperm32, p->d_xperm32, p->d_r32, p->d_d32,
+                               s));
+            k_add_f32<<<nblk(m), kT, 0, s>>>(p->d_d32, m, x);
+            RB_HIP(hipMemsetAsync(nrm + 1, 0, sizeof(unsigned long long), s));
+            k_residual<<<nblk(m), kT, 0, s>>>(p->d_rptr, p->d_rcol, p->d_rval,
+                                              p->d_levelItem, m, b, x, p->d_r,
+                                              nrm + 1);
+            RB_HIP(hipGetLastError());
+            unsigned long long hr = 0;
+            RB_HIP(hipMemcpyAsync(&hr, nrm + 1, sizeof(hr),
+                                  hipMemcpyDeviceToHost, s));
+            RB_HIP(hipStreamSynchronize(s));
+            rel = __builtin_bit_cast(double, hr) / bn;
+            if (rel <= tol) break;
+        }
+        if (!(rel <= tol)) bad = true;
+        it_max = std::max(it_max, it);
+        if (rel != rel || rel > rel_max) rel_max = rel;
+    }
+    if (iters) *iters = it_max;
+    if (relres) *relres = rel_max;
+    if (bad) {
+        set_error(RB_ERR_NOT_CONVERGED,
+                  "FP32 refinement did not reach the requested residual");
+        return RB_ERR_NOT_CONVERGED;
     }
     return RB_OK;
 }

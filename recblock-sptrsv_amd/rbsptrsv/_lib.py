@@ -20,14 +20,14 @@ EXPORTS = [
     "rb_plan_info_get",
     "rb_plan_blocks", "rb_plan_levels", "rb_plan_tri_levels",
     "rb_plan_block_csr", "rb_plan_permuted_csc", "rb_plan_trace",
-    "rb_plan_set_timing", "rb_plan_kernel_ms",
+    "rb_plan_set_timing", "rb_plan_kernel_ms", "rb_solve_f32ir",
 ]
 
 RB_OK = 0
 STATUS_NAMES = {0: "RB_OK", 1: "RB_ERR_INVALID_ARG", 2: "RB_ERR_NOT_SQUARE",
                 3: "RB_ERR_BAD_STRUCTURE", 4: "RB_ERR_MISSING_DIAG",
                 5: "RB_ERR_ZERO_DIAG", 6: "RB_ERR_HIP", 7: "RB_ERR_NO_MEMORY",
-                8: "RB_ERR_INTERNAL"}
+                8: "RB_ERR_INTERNAL", 9: "RB_ERR_NOT_CONVERGED"}
 
 SCHED_AUTO, SCHED_BLOCKED, SCHED_FUSED = 0, 1, 2
 
@@ -41,7 +41,7 @@ class RbError(RuntimeError):
 class rb_opts(ctypes.Structure):
     _fields_ = [("schedule", ctypes.c_int), ("device_id", ctypes.c_int),
                 ("fused_waves", ctypes.c_int), ("no_graph", ctypes.c_int),
-                ("reserved", ctypes.c_int * 4)]
+                ("mixed", ctypes.c_int), ("reserved", ctypes.c_int * 3)]
 
 
 class rb_plan_info(ctypes.Structure):

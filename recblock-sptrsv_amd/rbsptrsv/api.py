@@ -50,7 +50,7 @@ class Plan:
 
     def __init__(self, colptr, rowidx, val, substitution=0, lv=-1,
                  schedule=_lib.SCHED_AUTO, stream=None, device_id=0,
-                 fused_waves=0, graph=True):
+                 fused_waves=0, graph=True, mixed=False):
         torch = _torch()
         self._keep = []
         dcp = colptr if torch.is_tensor(colptr) else _dev_i32(colptr)
@@ -64,6 +64,7 @@ class Plan:
         opts.device_id = device_id
         opts.fused_waves = fused_waves
         opts.no_graph = 0 if graph else 1
+        opts.mixed = 1 if mixed else 0
         h = ctypes.c_void_p()
         check(lib().rb_plan_create(_ptr(dcp), _ptr(dri), _ptr(dv), self.m,
                                    self.m, self.nnz, substitution, lv,
@@ -120,6 +121,34 @@ class Plan:
         check(lib().rb_solve(self._h, _ptr(b), ldb, _ptr(x), ldx, nrhs,
                              _stream_ptr(stream)))
         return x
+
+    def solve_f32ir(self, b, x=None, tol=1e-12, max_iter=10, stream=None):
+        """FP32 solves with FP64 refinement (plan built with mixed=True).
+        b as in solve().  Returns (x, iterations, relative residual)."""
+        torch = _torch()
+        if not torch.is_tensor(b):
+            b = _dev_f64(b)
+        if b.dim() == 1:
+            nrhs, ldb = 1, self.m
+            if x is None:
+                x = torch.empty_like(b)
+            ldx = self.m
+        else:
+            nrhs = b.shape[1]
+            if b.stride(0) != 1:
+                raise ValueError("b must be column-major (b.t() contiguous)")
+            ldb = b.stride(1)
+            if x is None:
+                x = torch.empty((nrhs, self.m), dtype=b.dtype,
+                                device=b.device).t()
+            ldx = x.stride(1)
+        it = ctypes.c_int(0)
+        rr = ctypes.c_double(0.0)
+        check(lib().rb_solve_f32ir(self._h, _ptr(b), ldb, _ptr(x), ldx, nrhs,
+                                   ctypes.c_double(tol), max_iter,
+                                   _stream_ptr(stream), ctypes.byref(it),
+                                   ctypes.byref(rr)))
+        return x, it.value, rr.value
 
     def set_timing(self, enable=True):
         """Record events around each solve's main launch (see

@@ -367,3 +367,59 @@ def test_fused_many_rhs_match_single(oracle, torch, name, make):
             xo = oracle.solve_blocks(re, f.n, 2, f.substitution, part, B[j])
             assert rel_err(X[j], xo) < X_TOL
         assert max(residual(f, X[j], B[j]) for j in range(k)) < RES_TOL
+
+
+# ----------------------------------------------- FP32 + FP64 refinement
+@pytest.mark.parametrize("name,make", [
+    ("random2000", lambda: gen.random_lower(2000, 6, seed=2)),
+    ("laplacian40x30-upper", lambda: gen.laplacian(40, 30, lower=False)),
+    ("banded20000", lambda: gen.banded(20000, seed=5)),
+    ("powerlaw30000", lambda: gen.powerlaw(30000, seed=6)),
+])
+def test_f32_refinement_reaches_fp64_accuracy(oracle, torch, name, make):
+    f = make()
+    _say("f32ir", name)
+    p = _plan(f, 2, mixed=True)
+    b = gen.rhs(f.n, seed=4)
+    x, it, rel = p.solve_f32ir(torch.as_tensor(b, device="cuda"), tol=1e-12)
+    x = x.cpu().numpy()
+    p.check()
+    assert rel <= 1e-12 and 1 <= it <= 6, (it, rel)
+    # the reported max-norm residual is the true one, up to the rounding of
+    # the residual itself (another summation order on the host): a few ulps
+    # of max_i (|A| |x|)_i
+    import dataclasses
+    r = b - f.matvec(x)
+    absax = dataclasses.replace(f, val=np.abs(f.val)).matvec(np.abs(x))
+    slack = 64 * np.finfo(float).eps * absax.max() / np.abs(b).max()
+    assert abs(np.abs(r).max() / np.abs(b).max() - rel) <= slack
+    assert residual(f, x, b) < RES_TOL
+    re, part, bnz, an = oracle.full_pipeline(f.colptr, f.rowidx, f.val, f.n,
+                                             f.substitution, 2)
+    xo = oracle.solve_blocks(re, f.n, 2, f.substitution, part, b)
+    assert rel_err(x, xo) < X_TOL
+    # one FP32 correction alone is FP32-accurate only
+    from rbsptrsv import RbError
+    with pytest.raises(RbError) as ei:
+        p.solve_f32ir(torch.as_tensor(b, device="cuda"), tol=1e-14, max_iter=1)
+    assert ei.value.status == 9                      # not converged
+
+
+def test_f32_refinement_many_rhs_and_errors(torch):
+    f = gen.laplacian(50, 40)
+    p = _plan(f, 2, mixed=True)
+    k = 3
+    B = np.random.default_rng(3).uniform(-1, 1, (k, f.n))
+    B[1] = 0.0                                       # zero column: x = 0
+    Bd = torch.as_tensor(B, device="cuda").t()
+    X, it, rel = p.solve_f32ir(Bd)
+    X = X.t().cpu().numpy()
+    assert rel <= 1e-12
+    assert np.all(X[1] == 0.0)
+    for j in (0, 2):
+        assert residual(f, X[j], B[j]) < RES_TOL
+    q = _plan(f, 2)                                  # no FP32 copies
+    from rbsptrsv import RbError
+    with pytest.raises(RbError) as ei:
+        q.solve_f32ir(torch.as_tensor(B[0], device="cuda"))
+    assert ei.value.status == 1
