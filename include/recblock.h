@@ -207,6 +207,45 @@ rb_status rb_solve_f32ir(rb_plan *plan, const double *d_b, int ldb,
                          int max_iter, void *stream, int *iters,
                          double *relres);
 
+/* ---------------------------------------------------------------------
+ * Batches: many independent factors of uneven size, preprocessed as one
+ * block-diagonal plan, so that one persistent launch (fused schedule) or
+ * one captured block walk solves every factor and right-hand side.  No
+ * counterpart in the reference (one factor, one right-hand side).
+ * ------------------------------------------------------------------- */
+typedef struct rb_batch rb_batch;
+
+/* nfactors CSC factors (host arrays of device pointers and sizes), all of
+ * the same substitution; lv and opts apply to the union (lv = -1: auto
+ * rule on the total size).  Synchronises on stream. */
+rb_status rb_batch_create(int nfactors, const int *const *d_colptr,
+                          const int *const *d_rowidx,
+                          const double *const *d_val, const int *m,
+                          const int *nnz, int substitution, int lv,
+                          const rb_opts *opts, void *stream, rb_batch **batch);
+
+/* Factor f solves nrhs[f] right-hand sides d_b[f] (m[f] x nrhs[f],
+ * leading dimension ldb[f]) into d_x[f] (ldx[f]); nrhs[f] may be 0.
+ * Host arrays of device pointers.  Stream-ordered; the per-call tables are
+ * staged through pinned memory, so the arrays may be reused on return. */
+rb_status rb_batch_solve(rb_batch *batch, const double *const *d_b,
+                         const int *ldb, double *const *d_x, const int *ldx,
+                         const int *nrhs, void *stream);
+
+/* The same with FP32 solves and FP64 refinement (opts->mixed = 1 at
+ * creation); see rb_solve_f32ir. */
+rb_status rb_batch_solve_f32ir(rb_batch *batch, const double *const *d_b,
+                               const int *ldb, double *const *d_x,
+                               const int *ldx, const int *nrhs, double tol,
+                               int max_iter, void *stream, int *iters,
+                               double *relres);
+
+/* The union plan (for rb_plan_info_get, timing and inspection); owned by
+ * the batch. */
+rb_plan *rb_batch_plan(rb_batch *batch);
+
+rb_status rb_batch_destroy(rb_batch *batch);
+
 /* Kernel timing for measurement harnesses.  While enabled, every solve
  * records HIP events on its stream around its main launch: the fused
  * kernel (fused schedule) or the captured block walk (blocked schedule),

@@ -7,5 +7,5 @@ workload generators.
 from . import gen  # noqa: F401
 from ._lib import (EXPORTS, LIB_PATH, RbError, SCHED_AUTO, SCHED_BLOCKED,  # noqa: F401
                    SCHED_FUSED, build, lib)
-from .api import (Plan, lv_auto, recblocking_solver,  # noqa: F401
+from .api import (Batch, Plan, lv_auto, recblocking_solver,  # noqa: F401
                   recblocking_solver_cuda, set_bench_repeat)

@@ -21,6 +21,8 @@ EXPORTS = [
     "rb_plan_blocks", "rb_plan_levels", "rb_plan_tri_levels",
     "rb_plan_block_csr", "rb_plan_permuted_csc", "rb_plan_trace",
     "rb_plan_set_timing", "rb_plan_kernel_ms", "rb_solve_f32ir",
+    "rb_batch_create", "rb_batch_solve", "rb_batch_solve_f32ir",
+    "rb_batch_plan", "rb_batch_destroy",
 ]
 
 RB_OK = 0
@@ -103,6 +105,29 @@ def lib():
     L.rb_plan_permuted_csc.restype = i
     L.rb_plan_trace.argtypes = [vp, vp, ctypes.POINTER(i)]
     L.rb_plan_trace.restype = i
+    L.rb_plan_set_timing.argtypes = [vp, i]
+    L.rb_plan_set_timing.restype = i
+    L.rb_plan_kernel_ms.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
+    L.rb_plan_kernel_ms.restype = i
+    L.rb_solve_f32ir.argtypes = [vp, vp, i, vp, i, i, ctypes.c_double, i, vp,
+                                 ctypes.POINTER(i),
+                                 ctypes.POINTER(ctypes.c_double)]
+    L.rb_solve_f32ir.restype = i
+    L.rb_batch_create.argtypes = [i, vp, vp, vp, vp, vp, i, i,
+                                  ctypes.POINTER(rb_opts), vp,
+                                  ctypes.POINTER(vp)]
+    L.rb_batch_create.restype = i
+    L.rb_batch_solve.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    L.rb_batch_solve.restype = i
+    L.rb_batch_solve_f32ir.argtypes = [vp, vp, vp, vp, vp, vp,
+                                       ctypes.c_double, i, vp,
+                                       ctypes.POINTER(i),
+                                       ctypes.POINTER(ctypes.c_double)]
+    L.rb_batch_solve_f32ir.restype = i
+    L.rb_batch_plan.argtypes = [vp]
+    L.rb_batch_plan.restype = vp
+    L.rb_batch_destroy.argtypes = [vp]
+    L.rb_batch_destroy.restype = i
     L.recblocking_solver_cuda.argtypes = [vp, vp, vp, i, i, i, vp, vp, i, i,
                                           ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_double)]

@@ -225,6 +225,117 @@ class Plan:
         return cp, ri[:self.nnz], vv[:self.nnz]
 
 
+def _colmajor(torch, b, m):
+    """(pointer, ld, k) of a (m,) or column-major (m, k) device tensor."""
+    if b.dim() == 1:
+        return _ptr(b), m, 1
+    if b.stride(0) != 1:
+        raise ValueError("right-hand sides must be column-major (b.t() contiguous)")
+    return _ptr(b), b.stride(1), b.shape[1]
+
+
+class Batch:
+    """Many independent factors preprocessed as one block-diagonal plan and
+    solved together (rb_batch_*).  ``factors``: objects with ``colptr``,
+    ``rowidx``, ``val`` (host arrays or device tensors) and
+    ``substitution`` (all equal)."""
+
+    def __init__(self, factors, lv=-1, schedule=_lib.SCHED_AUTO, stream=None,
+                 device_id=0, fused_waves=0, mixed=False):
+        torch = _torch()
+        if not factors:
+            raise ValueError("empty batch")
+        subst = factors[0].substitution
+        if any(f.substitution != subst for f in factors):
+            raise ValueError("all factors of a batch share one substitution")
+        self._keep = []
+        cps, ris, vs, ms, zs = [], [], [], [], []
+        for f in factors:
+            cp = f.colptr if torch.is_tensor(f.colptr) else _dev_i32(f.colptr)
+            ri = f.rowidx if torch.is_tensor(f.rowidx) else _dev_i32(f.rowidx)
+            v = f.val if torch.is_tensor(f.val) else _dev_f64(f.val)
+            self._keep += [cp, ri, v]
+            cps.append(cp.data_ptr())
+            ris.append(ri.data_ptr())
+            vs.append(v.data_ptr())
+            ms.append(int(cp.numel()) - 1)
+            zs.append(int(ri.numel()))
+        nf = len(factors)
+        P = ctypes.c_void_p * nf
+        I = ctypes.c_int * nf
+        opts = _lib.rb_opts()
+        lib().rb_opts_default(ctypes.byref(opts))
+        opts.schedule = schedule
+        opts.device_id = device_id
+        opts.fused_waves = fused_waves
+        opts.mixed = 1 if mixed else 0
+        h = ctypes.c_void_p()
+        check(lib().rb_batch_create(nf, P(*cps), P(*ris), P(*vs), I(*ms),
+                                    I(*zs), subst, lv, ctypes.byref(opts),
+                                    _stream_ptr(stream), ctypes.byref(h)))
+        self._h = h
+        self.m = ms
+        self.nf = nf
+        self.substitution = subst
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().rb_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        inf = _lib.rb_plan_info()
+        check(lib().rb_plan_info_get(lib().rb_batch_plan(self._h),
+                                     ctypes.byref(inf)))
+        return inf
+
+    def _tables(self, bs, xs):
+        torch = _torch()
+        nf = self.nf
+        if xs is None:
+            xs = [None if b is None else
+                  (torch.empty_like(b) if b.dim() == 1 else
+                   torch.empty((b.shape[1], b.shape[0]), dtype=b.dtype,
+                               device=b.device).t()) for b in bs]
+        bp, xp, lb, lx, nr = [], [], [], [], []
+        for f in range(nf):
+            if bs[f] is None:
+                bp.append(None), xp.append(None)
+                lb.append(self.m[f]), lx.append(self.m[f]), nr.append(0)
+                continue
+            pb, ldb, k = _colmajor(torch, bs[f], self.m[f])
+            px, ldx, _ = _colmajor(torch, xs[f], self.m[f])
+            bp.append(pb.value), xp.append(px.value)
+            lb.append(ldb), lx.append(ldx), nr.append(k)
+        P = ctypes.c_void_p * nf
+        I = ctypes.c_int * nf
+        return xs, (P(*bp), I(*lb), P(*xp), I(*lx), I(*nr))
+
+    def solve(self, bs, xs=None, stream=None):
+        """bs[f]: factor f's right-hand sides, (m_f,) or column-major
+        (m_f, k_f) device tensor, or None.  Returns the list of solutions."""
+        xs, t = self._tables(bs, xs)
+        check(lib().rb_batch_solve(self._h, *t, _stream_ptr(stream)))
+        return xs
+
+    def solve_f32ir(self, bs, xs=None, tol=1e-12, max_iter=10, stream=None):
+        """FP32 solves with FP64 refinement (batch built with mixed=True).
+        Returns (solutions, iterations, relative residual)."""
+        xs, t = self._tables(bs, xs)
+        it = ctypes.c_int(0)
+        rr = ctypes.c_double(0.0)
+        check(lib().rb_batch_solve_f32ir(self._h, *t, ctypes.c_double(tol),
+                                         max_iter, _stream_ptr(stream),
+                                         ctypes.byref(it), ctypes.byref(rr)))
+        return xs, it.value, rr.value
+
+
 def recblocking_solver_cuda(d_cscColPtrTR, d_cscRowIdxTR, d_cscValTR, m, n,
                             nnzTR, d_x, d_b, substitution, lv):
     """Reference entry point (R/recblocking_solver_cuda.h:21-32) on torch

@@ -423,3 +423,63 @@ def test_f32_refinement_many_rhs_and_errors(torch):
     with pytest.raises(RbError) as ei:
         q.solve_f32ir(torch.as_tensor(B[0], device="cuda"))
     assert ei.value.status == 1
+
+
+# ------------------------------------------------------------ batches
+def _small_batch():
+    sizes = [300, 1200, 50, 2500, 800, 1]
+    facs = [gen.banded(n, seed=10 + j, window=min(200, max(n - 1, 1)))
+            if n > 1 else csc_from_deps(1, {}, diag=3.0)
+            for j, n in enumerate(sizes)]
+    facs[2] = gen.random_lower(50, 4, seed=3)
+    return facs, [2, 0, 1, 5, 3, 1]
+
+
+@pytest.mark.parametrize("schedule", [0, 1, 2])
+def test_batch_matches_single_factor_solves(oracle, torch, schedule):
+    from rbsptrsv import Batch
+    facs, nrhs = _small_batch()
+    _say("batch", schedule)
+    bt = Batch(facs, lv=2, schedule=schedule)
+    rng = np.random.default_rng(11)
+    Bs = [rng.uniform(-1, 1, (k, f.n)) for f, k in zip(facs, nrhs)]
+    bs = [None if k == 0 else
+          (torch.as_tensor(B[0], device="cuda") if k == 1 else
+           torch.as_tensor(B, device="cuda").t()) for B, k in zip(Bs, nrhs)]
+    xs = bt.solve(bs)
+    torch.cuda.synchronize()
+    for f, B, k, x in zip(facs, Bs, nrhs, xs):
+        if k == 0:
+            assert x is None
+            continue
+        X = x.cpu().numpy()[None, :] if k == 1 else x.t().cpu().numpy()
+        lv = 2 if f.n >= 4 else 0
+        re, part, bnz, an = oracle.full_pipeline(f.colptr, f.rowidx, f.val,
+                                                 f.n, 0, lv)
+        for j in range(k):
+            xo = oracle.solve_blocks(re, f.n, lv, 0, part, B[j])
+            assert rel_err(X[j], xo) < X_TOL
+            assert residual(f, X[j], B[j]) < RES_TOL
+    # the same solve again is bitwise the same
+    xs2 = bt.solve(bs)
+    for x, x2 in zip(xs, xs2):
+        if x is not None:
+            assert torch.equal(x, x2)
+
+
+def test_batch_f32_refinement(torch):
+    from rbsptrsv import Batch
+    facs, nrhs = _small_batch()
+    bt = Batch(facs, lv=2, mixed=True)
+    rng = np.random.default_rng(12)
+    bs = [None if k == 0 else torch.as_tensor(rng.uniform(-1, 1, (k, f.n)),
+                                              device="cuda").t()
+          for f, k in zip(facs, nrhs)]
+    xs, it, rel = bt.solve_f32ir(bs, tol=1e-12)
+    assert rel <= 1e-12 and it >= 1
+    for f, b, x in zip(facs, bs, xs):
+        if b is None:
+            continue
+        B, X = b.t().cpu().numpy(), x.t().cpu().numpy()
+        for j in range(B.shape[0]):
+            assert residual(f, X[j], B[j]) < RES_TOL
