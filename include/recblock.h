@@ -57,7 +57,9 @@ typedef struct {
     int mixed;         /* 1 = also keep FP32 copies of the block values and
                           the FP64 matrix for residuals: enables
                           rb_solve_f32ir */
-    int reserved[3];
+    int tune_rhs;      /* RB_SCHED_AUTO: right-hand sides per call the
+                          schedule is tuned for (0 or 1: single) */
+    int reserved[2];
 } rb_opts;
 
 typedef struct rb_plan rb_plan;

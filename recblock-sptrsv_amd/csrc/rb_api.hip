@@ -41,9 +41,9 @@ static int lv_auto(int m, int device_id) {
     return li;
 }
 
-// Schedule choice.  BLOCKED and FUSED are taken as asked; AUTO times one
-// warm solve of each candidate on a scratch right-hand side and keeps the
-// faster (the plan is built for both).
+// Schedule choice.  BLOCKED and FUSED are taken as asked; AUTO times warm
+// solves of each candidate on scratch right-hand sides (tune_rhs columns
+// per call, default 1) and keeps the faster (the plan is built for both).
 static rb_status choose_schedule(rb_plan *p, hipStream_t s) {
     if (p->want_schedule == RB_SCHED_BLOCKED || !p->fused) {
         p->schedule = RB_SCHED_BLOCKED;
@@ -54,25 +54,25 @@ static rb_status choose_schedule(rb_plan *p, hipStream_t s) {
         return RB_OK;
     }
     const int m = p->m;
+    const int k = std::max(1, std::min(p->tune_rhs, 64));
+    const size_t bytes = sizeof(double) * (size_t)m * k;
     double *db = nullptr, *dx = nullptr;
-    RB_HIP(hipMalloc(&db, sizeof(double) * (size_t)m));
-    if (hipMalloc(&dx, sizeof(double) * (size_t)m) != hipSuccess) {
+    RB_HIP(hipMalloc(&db, bytes));
+    if (hipMalloc(&dx, bytes) != hipSuccess) {
         (void)hipFree(db);
         return status_of_hip(hipGetLastError(), "hipMalloc(tune)");
     }
-    std::vector<double> ones((size_t)m, 1.0);
     rb_status st = RB_OK;
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
-    if (hipMemcpy(db, ones.data(), sizeof(double) * m, hipMemcpyHostToDevice) !=
-        hipSuccess)
-        st = RB_ERR_HIP;
+    // the work of a solve does not depend on the values: b = 0
+    if (hipMemsetAsync(db, 0, bytes, s) != hipSuccess) st = RB_ERR_HIP;
     double ms[2] = {0.0, 0.0};
     for (int c = 0; c < 2 && st == RB_OK; c++) {
         auto run = [&]() {
-            return c == 0 ? solve_blocked(p, db, m, dx, m, 1, s)
-                          : solve_fused(p, db, m, dx, m, 1, s);
+            return c == 0 ? solve_blocked(p, db, m, dx, m, k, s)
+                          : solve_fused(p, db, m, dx, m, k, s);
         };
         st = run();  // warm
         if (st != RB_OK) break;
@@ -167,6 +167,7 @@ rb_status rb_plan_create(const int *d_colptr, const int *d_rowidx,
     p->fused_waves = o.fused_waves;
     p->graph_failed = o.no_graph != 0;  // eager launches only
     p->mixed = o.mixed != 0;
+    p->tune_rhs = o.tune_rhs;
     rb_status st = build_plan(p, d_colptr, d_rowidx, d_val,
                               (hipStream_t)stream);
     if (st == RB_OK) st = choose_schedule(p, (hipStream_t)stream);

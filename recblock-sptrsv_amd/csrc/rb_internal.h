@@ -143,6 +143,7 @@ struct rb_plan {
     int want_schedule = RB_SCHED_AUTO;
     int fused_waves = 0;        // wave cap for the fused partition (0 = auto)
     double tune_ms_blocked = 0.0, tune_ms_fused = 0.0;
+    int tune_rhs = 1;           // right-hand sides per tuning solve
     // blocked schedule: the block walk captured once as a graph
     void *graph_exec = nullptr;
     bool graph_failed = false;

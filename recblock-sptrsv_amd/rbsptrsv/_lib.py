@@ -43,7 +43,8 @@ class RbError(RuntimeError):
 class rb_opts(ctypes.Structure):
     _fields_ = [("schedule", ctypes.c_int), ("device_id", ctypes.c_int),
                 ("fused_waves", ctypes.c_int), ("no_graph", ctypes.c_int),
-                ("mixed", ctypes.c_int), ("reserved", ctypes.c_int * 3)]
+                ("mixed", ctypes.c_int), ("tune_rhs", ctypes.c_int),
+                ("reserved", ctypes.c_int * 2)]
 
 
 class rb_plan_info(ctypes.Structure):

@@ -50,7 +50,7 @@ class Plan:
 
     def __init__(self, colptr, rowidx, val, substitution=0, lv=-1,
                  schedule=_lib.SCHED_AUTO, stream=None, device_id=0,
-                 fused_waves=0, graph=True, mixed=False):
+                 fused_waves=0, graph=True, mixed=False, tune_rhs=1):
         torch = _torch()
         self._keep = []
         dcp = colptr if torch.is_tensor(colptr) else _dev_i32(colptr)
@@ -65,6 +65,7 @@ class Plan:
         opts.fused_waves = fused_waves
         opts.no_graph = 0 if graph else 1
         opts.mixed = 1 if mixed else 0
+        opts.tune_rhs = tune_rhs
         h = ctypes.c_void_p()
         check(lib().rb_plan_create(_ptr(dcp), _ptr(dri), _ptr(dv), self.m,
                                    self.m, self.nnz, substitution, lv,
@@ -241,7 +242,7 @@ class Batch:
     ``substitution`` (all equal)."""
 
     def __init__(self, factors, lv=-1, schedule=_lib.SCHED_AUTO, stream=None,
-                 device_id=0, fused_waves=0, mixed=False):
+                 device_id=0, fused_waves=0, mixed=False, tune_rhs=1):
         torch = _torch()
         if not factors:
             raise ValueError("empty batch")
@@ -269,6 +270,7 @@ class Batch:
         opts.device_id = device_id
         opts.fused_waves = fused_waves
         opts.mixed = 1 if mixed else 0
+        opts.tune_rhs = tune_rhs
         h = ctypes.c_void_p()
         check(lib().rb_batch_create(nf, P(*cps), P(*ris), P(*vs), I(*ms),
                                     I(*zs), subst, lv, ctypes.byref(opts),
