@@ -178,6 +178,85 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def serve_factors():
+    """16 resident factors, n log-spaced in [1e4, 4e6], recipes cycling
+    banded / Laplacian lower / power-law."""
+    from rbsptrsv import gen
+    facs = []
+    for i in range(16):
+        n = int(round(1e4 * 400 ** (i / 15)))
+        if i % 3 == 0:
+            facs.append(gen.banded(n, seed=100 + i))
+        elif i % 3 == 1:
+            side = int(round(n ** 0.5))
+            facs.append(gen.laplacian(side))
+        else:
+            facs.append(gen.powerlaw(n, seed=100 + i))
+    return facs
+
+
+def bench_serve(args):
+    """Poisson request stream over 16 resident factors (Zipf(1.1) choice),
+    served with continuous cross-request batching; the same stream served
+    one request per solve for comparison."""
+    import numpy as np
+    from rbsptrsv.serve import Server, poisson_requests
+
+    def say(*a):
+        print("[serve16]", *a, file=sys.stderr, flush=True)
+
+    facs = serve_factors()
+    say("factors", [f.n for f in facs])
+    sizes = [f.n for f in facs]
+    reqs = poisson_requests(16, sizes, args.requests, rate=args.rate, seed=9)
+    res = {}
+    modes = (("batched_fp64", "fp64", 64), ("unbatched_fp64", "fp64", 1),
+             ("batched_f32ir", "f32ir", 64))
+    if args.serve_modes:
+        modes = [m for m in modes if m[0] in args.serve_modes.split(",")]
+    for name, mode, mb in modes:
+        t0 = time.perf_counter()
+        srv = Server(facs, mode=mode, max_batch=mb, streams=4,
+                     mem_budget=8 << 30)
+        build_s = time.perf_counter() - t0
+        say(name, "server ready in %.1f s" % build_s)
+        stats, out = srv.run(reqs)
+        for p in srv.plans:
+            p.check()
+        rels = []
+        for r in out[:: max(1, len(out) // 200)]:
+            f = facs[r.factor]
+            b = np.asarray(reqs[r.rid][1])
+            x = r.x.cpu().numpy()
+            rels.append(float(np.linalg.norm(b - f.matvec(x)) /
+                              np.linalg.norm(b)))
+        rels = np.array(rels)
+        stats["residual_sampled_max"] = float(rels.max())   # NaN if any is
+        stats["residual_sampled_nonfinite"] = int((~np.isfinite(rels)).sum())
+        stats["server_build_s"] = round(build_s, 1)
+        res[name] = {k: (round(v, 4) if isinstance(v, float) and
+                         not k.startswith("residual") else v)
+                     for k, v in stats.items()}
+        say(name, res[name])
+        srv.close()
+    b = res.get("batched_fp64") or next(iter(res.values()))
+    out = {
+        "metric": BASELINE_METRIC,
+        "value": round(b["solves_per_s"], 1),
+        "unit": "solves/s",
+        "n_gpus": 1, "steps": args.requests, "warmup": 1,
+        "ms_per_step": round(1e3 / b["solves_per_s"], 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded generator)",
+        "config": {"workload": "serve-16-factors-1e4..4e6-poisson",
+                   "offered_rate_per_s": args.rate,
+                   "requests": args.requests, "zipf": 1.1},
+        "p50_ms": b["p50_ms"], "p99_ms": b["p99_ms"],
+        "servers": res,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -192,6 +271,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true",
                     help="batch workload: skip the one-plan-per-factor path")
+    ap.add_argument("--rate", type=float, default=4000.0,
+                    help="serve workload: offered requests per second")
+    ap.add_argument("--requests", type=int, default=3000,
+                    help="serve workload: requests in the stream")
+    ap.add_argument("--serve-modes", default="",
+                    help="serve workload: comma list of servers to run")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only (this build has no multi-GPU "
@@ -200,6 +285,8 @@ def main():
 
     if args.workload == "batch64":
         return bench_batch(args)
+    if args.workload == "serve16":
+        return bench_serve(args)
 
     import torch
     from rbsptrsv import Plan, gen

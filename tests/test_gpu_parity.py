@@ -483,3 +483,41 @@ def test_batch_f32_refinement(torch):
         B, X = b.t().cpu().numpy(), x.t().cpu().numpy()
         for j in range(B.shape[0]):
             assert residual(f, X[j], B[j]) < RES_TOL
+
+
+# ------------------------------------------------------------ serving
+@pytest.mark.parametrize("mode", ["fp64", "f32ir"])
+def test_serving_results_same_alone_and_in_the_mix(torch, mode):
+    """Requests served with continuous cross-request batching give the
+    result of the same request solved alone: bitwise for fp64, and within
+    the refinement tolerance for f32ir; a memory budget of one request at a
+    time still completes everything."""
+    from rbsptrsv.serve import Server, poisson_requests
+    facs = [gen.banded(3000, seed=1, window=300), gen.laplacian(40, 40),
+            gen.random_lower(2500, 5, seed=4)]
+    sizes = [f.n for f in facs]
+    reqs = poisson_requests(len(facs), sizes, 60, rate=2e4, seed=2)
+    srv = Server(facs, mode=mode, max_batch=16, streams=2)
+    stats, out = srv.run(reqs)
+    _say("serve", mode, stats)
+    assert stats["requests"] == 60 and stats["p99_ms"] >= stats["p50_ms"] > 0
+    assert stats["mean_batch"] > 1.0          # requests were batched
+    for r in out:
+        f = facs[r.factor]
+        x = r.x.cpu().numpy()
+        assert residual(f, x, r.b.cpu().numpy()) < RES_TOL
+        alone = srv.plans[r.factor]
+        bd = torch.as_tensor(np.asarray(reqs[r.rid][1]), device="cuda")
+        if mode == "fp64":
+            xa = alone.solve(bd).cpu().numpy()
+            assert np.array_equal(x.view(np.int64), xa.view(np.int64))
+        else:
+            xa, it, rel = alone.solve_f32ir(bd)
+            assert rel_err(x, xa.cpu().numpy()) < 1e-10
+    tight = Server(facs, mode="fp64", max_batch=16, streams=2,
+                   mem_budget=16 * 3000)
+    stats2, out2 = tight.run(reqs[:20])
+    assert stats2["requests"] == 20
+    for r in out2:
+        assert residual(facs[r.factor], r.x.cpu().numpy(),
+                        r.b.cpu().numpy()) < RES_TOL
