@@ -75,7 +75,7 @@ def cpu_baseline(f, lv, budget_s):
         oracle.solve_blocks(re, f.n, lv, f.substitution, part, b)
         reps += 1
         el = time.perf_counter() - t0
-        if el >= budget_s or reps >= 200:
+        if el >= budget_s or reps >= 2000:
             break
     per = el / reps
     return dict(value=round(algorithmic_bytes(f.n, f.nnz) / per / 1e9, 4),
