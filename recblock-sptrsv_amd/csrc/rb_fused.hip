@@ -267,25 +267,27 @@ struct Consumer {
             double bv;
             if constexpr (SEPB) bv = bb[U];
             else bv = dbl(f[4], f[5]);
-            double acc = 0.0;
+            // everything that does not depend on the previous step's x
+            // first, so the chain that carries it is DPP + 3 FMAs + 2 ops
             const int gtot = __builtin_amdgcn_readlane(f[1], 63) +
                              (__builtin_amdgcn_readlane(flags, 63) & kHasGlobal);
             if (gtot > 0) {
                 if (fifo_avail < fifo_pos + gtot) wait_fifo(fifo_pos + gtot);
                 const double fv = fifo[(fifo_pos + f[1]) & (kFF - 1)];
-                acc = (flags & kHasGlobal) ? fv : 0.0;
+                bv -= (flags & kHasGlobal) ? fv : 0.0;
                 fifo_pos += gtot;
             }
 #pragma unroll
             for (int j = 0; j < NRING; j++) {
                 const double xv =
                     *(const double *)((const char *)xring + f[F::kSrc + j]);
-                acc += dbl(f[F::kVal + 2 * j], f[F::kVal + 2 * j + 1]) * xv;
+                bv -= dbl(f[F::kVal + 2 * j], f[F::kVal + 2 * j + 1]) * xv;
             }
+            double acc = 0.0;
             if constexpr (NDPP > 0) {
                 const double xl = dpp_from_lower(xprev);
                 const double xr = dpp_from_upper(xprev);
-                acc += dbl(f[6], f[7]) * xprev;
+                acc = dbl(f[6], f[7]) * xprev;
                 acc += dbl(f[8], f[9]) * xl;
                 acc += dbl(f[10], f[11]) * xr;
             }
