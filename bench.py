@@ -265,6 +265,8 @@ def main():
     ap.add_argument("--workload", default="laplacian")
     ap.add_argument("--schedule", type=int, default=0,
                     help="0 auto, 1 blocked, 2 fused")
+    ap.add_argument("--fused-waves", type=int, default=0,
+                    help="cap on the fused schedule's consumer waves (0 auto)")
     ap.add_argument("--rhs", type=int, default=1,
                     help="right-hand sides per solve call (column-major)")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
@@ -296,7 +298,8 @@ def main():
     b = gen.rhs(f.n, seed=1)
     stream = torch.cuda.current_stream()
     t0 = time.perf_counter()
-    plan = Plan.from_factor(f, lv=-1, schedule=args.schedule)
+    plan = Plan.from_factor(f, lv=-1, schedule=args.schedule,
+                            fused_waves=args.fused_waves)
     pre_ms = (time.perf_counter() - t0) * 1e3
     info = plan.info()
     if K == 1:

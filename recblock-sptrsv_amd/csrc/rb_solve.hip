@@ -14,9 +14,11 @@
 //             FP atomics of R/utils_sptrsv_cuda.h:53-114, and cuSPARSE
 //             csrsv2 for method 1, which ignores the stored diagonal)
 // and a square block one SpMV with the subtraction b -= y fused in
-//   method 0/2 CSR thread / wave per row, 1/3 DCSR thread / wave per
-//   non-empty row (R/utils_spmv_cuda.h:60-203), plus a workgroup per row
-//   longer than 2048 entries (the long-row pass :30-58).
+//   method 0/2 over all rows (CSR), 1/3 over the non-empty rows (DCSR)
+//   (R/utils_spmv_cuda.h:60-203), with G lanes per row, G the power of two
+//   at or above the block's mean row length (the reference uses one thread
+//   or one warp), plus a workgroup per row longer than 2048 entries (the
+//   long-row pass :30-58).
 // Every sum runs in a fixed order, so a solve is bitwise reproducible.
 //
 // The kernels are templates on the value type: the FP64 walk is the plain
@@ -265,6 +267,28 @@ __global__ void k_sq_dcsr_wave(const int *__restrict__ rowptr,
     if (lane == 0) bperm[up + r] -= acc;
 }
 
+// square block SpMV with G lanes per row (G a power of two, 2..32): the
+// group reads its row's entries coalesced, then a fixed-order butterfly;
+// rows from the DCSR list (non-empty rows) or all rows of the block
+template <class T, int G, bool DCSR>
+__global__ void __launch_bounds__(kT)
+k_sq_group(const int *__restrict__ rowptr, const int *__restrict__ col,
+           const T *__restrict__ val, const T *__restrict__ xperm,
+           const int *__restrict__ rows, int nrows, int sb, int up, int left,
+           T *bperm) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = t / G, g = t & (G - 1);
+    if (k >= nrows) return;  // whole groups leave together
+    const int r = DCSR ? rows[k] : k;
+    const int s = rowptr[sb + r], e = rowptr[sb + r + 1];
+    if (s == e || e - s > kLongRow) return;
+    T acc = 0;
+    for (int j = s + g; j < e; j += G) acc += val[j] * xperm[left + col[j]];
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, G);
+    if (g == 0) bperm[up + r] -= acc;
+}
+
 // one workgroup per long row (> 2048 entries), fixed-order tree reduction
 template <class T>
 __global__ void __launch_bounds__(kT)
@@ -288,6 +312,28 @@ k_sq_long(const int *__restrict__ rowptr, const int *__restrict__ col,
 }
 
 inline int nblk(long n, int t = kT) { return (int)((n + t - 1) / t); }
+
+// lanes per row for a square block's SpMV: the power of two at or above
+// its mean non-empty row length (long rows excluded), 1 .. 64
+inline int lanes_per_row(int nnzr) {
+    int g = 1;
+    while (g < nnzr && g < 64) g <<= 1;
+    return g;
+}
+
+template <class T, bool DCSR>
+void launch_sq(int g, int nrows, hipStream_t s, const int *rowptr,
+               const int *col, const T *val, const T *xperm, const int *rows,
+               int sb, int up, int left, T *bperm) {
+    const long th = (long)nrows * g;
+    switch (g) {
+    case 2: k_sq_group<T, 2, DCSR><<<nblk(th), kT, 0, s>>>(rowptr, col, val, xperm, rows, nrows, sb, up, left, bperm); break;
+    case 4: k_sq_group<T, 4, DCSR><<<nblk(th), kT, 0, s>>>(rowptr, col, val, xperm, rows, nrows, sb, up, left, bperm); break;
+    case 8: k_sq_group<T, 8, DCSR><<<nblk(th), kT, 0, s>>>(rowptr, col, val, xperm, rows, nrows, sb, up, left, bperm); break;
+    case 16: k_sq_group<T, 16, DCSR><<<nblk(th), kT, 0, s>>>(rowptr, col, val, xperm, rows, nrows, sb, up, left, bperm); break;
+    default: k_sq_group<T, 32, DCSR><<<nblk(th), kT, 0, s>>>(rowptr, col, val, xperm, rows, nrows, sb, up, left, bperm); break;
+    }
+}
 
 // ---------------------------------------------------- refinement kernels
 __global__ void k_to_f32(const double *__restrict__ a, int m, float *o) {
@@ -378,30 +424,39 @@ static void launch_blocks(rb_plan *p, const T *val, const T *invd, T *bperm,
                 break;
             }
         } else {
+            // The reference's method picks CSR (0, 2) or DCSR (1, 3) rows;
+            // lanes per row follow the block's mean row length (the
+            // reference's thread / warp split is 1 or a whole wave).
             const int *rows = p->d_dcsr_rows + B.dcsr_base;
-            switch (B.method) {
-            case -1:
-                break;
-            case 0:
-                k_sq_csr_thread<T><<<nblk(B.blk_m), kT, 0, s>>>(
-                    p->d_rowptr, p->d_col, val, xperm, B.slot_base, B.up,
-                    B.left, B.blk_m, bperm);
-                break;
-            case 2:
-                k_sq_csr_wave<T><<<nblk((long)B.blk_m * 64), kT, 0, s>>>(
-                    p->d_rowptr, p->d_col, val, xperm, B.slot_base, B.up,
-                    B.left, B.blk_m, bperm);
-                break;
-            case 1:
-                k_sq_dcsr_thread<T><<<nblk(B.m_new), kT, 0, s>>>(
-                    p->d_rowptr, p->d_col, val, xperm, rows, B.m_new,
-                    B.slot_base, B.up, B.left, bperm);
-                break;
-            default:
-                k_sq_dcsr_wave<T><<<nblk((long)B.m_new * 64), kT, 0, s>>>(
-                    p->d_rowptr, p->d_col, val, xperm, rows, B.m_new,
-                    B.slot_base, B.up, B.left, bperm);
-                break;
+            const bool dcsr = B.method == 1 || B.method == 3;
+            const int nr = dcsr ? B.m_new : B.blk_m;
+            const int g = lanes_per_row(B.nnzr);
+            if (B.method < 0 || nr == 0) {
+                // empty block
+            } else if (g == 1) {
+                if (dcsr)
+                    k_sq_dcsr_thread<T><<<nblk(nr), kT, 0, s>>>(
+                        p->d_rowptr, p->d_col, val, xperm, rows, nr,
+                        B.slot_base, B.up, B.left, bperm);
+                else
+                    k_sq_csr_thread<T><<<nblk(nr), kT, 0, s>>>(
+                        p->d_rowptr, p->d_col, val, xperm, B.slot_base, B.up,
+                        B.left, nr, bperm);
+            } else if (g == 64) {
+                if (dcsr)
+                    k_sq_dcsr_wave<T><<<nblk((long)nr * 64), kT, 0, s>>>(
+                        p->d_rowptr, p->d_col, val, xperm, rows, nr,
+                        B.slot_base, B.up, B.left, bperm);
+                else
+                    k_sq_csr_wave<T><<<nblk((long)nr * 64), kT, 0, s>>>(
+                        p->d_rowptr, p->d_col, val, xperm, B.slot_base, B.up,
+                        B.left, nr, bperm);
+            } else if (dcsr) {
+                launch_sq<T, true>(g, nr, s, p->d_rowptr, p->d_col, val, xperm,
+                                   rows, B.slot_base, B.up, B.left, bperm);
+            } else {
+                launch_sq<T, false>(g, nr, s, p->d_rowptr, p->d_col, val, xperm,
+                                    rows, B.slot_base, B.up, B.left, bperm);
             }
             if (B.method >= 0 && B.longrow > 0)
                 k_sq_long<T><<<B.longrow, kT, 0, s>>>(
