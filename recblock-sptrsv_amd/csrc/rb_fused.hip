@@ -39,6 +39,7 @@
 #include <algorithm>
 #include <array>
 #include <chrono>
+#include <cstdio>
 #include <cstring>
 #include <numeric>
 #include <vector>
@@ -50,20 +51,23 @@ namespace rb {
 namespace {
 
 constexpr int kR = 32;          // LDS x ring depth (steps), a power of two
-constexpr int kFF = 1024;       // FIFO capacity (groups), a power of two
+constexpr int kFF = 2048;       // FIFO capacity (groups), a power of two
 constexpr int kU = 2;           // resolver: batches of 64 polled together
-constexpr int kSigLag = 128;    // consumer reports FIFO use at this lag
 constexpr unsigned kSpin = 1u << 22;
 constexpr int kZeroOff = kR * 64 * 8;  // byte offset of the ring's zero
+constexpr int kWG = 128;        // consumer wave + resolver wave
 
 // record flags (int 0 of a lane's record)
 constexpr int kHasGlobal = 1;      // the row has a global group
-// word 1: group offset (groups of lower lanes in this step); lane 63's
-// offset plus its own group is the step's group count
+// bits 1.. of lane 0's flags in the first step of a block: the block's
+// group count
+constexpr int kBlockShift = 1;
+// int 1: the FIFO offset of the lane's group from the block's first group
+// (groups of the block's earlier steps and of lower lanes in this step)
 
 // Per-lane record of one step, as 32-bit words; every double starts at an
 // even word so it is an aligned register pair of the load that brings it:
-//   [0] flags  [1] group offset  [2..3] 1/diag  [4..5] b (per solve)
+//   [0] flags  [1] block group offset  [2..3] 1/diag  [4..5] b (per solve)
 //   NDPP: [6..7] coefficient of lane l (previous step), [8..9] of lane l-1,
 //         [10..11] of lane l+1
 //   NRING ring sources (byte offsets into the ring), padded to even, then
@@ -72,11 +76,15 @@ constexpr int kHasGlobal = 1;      // the row has a global group
 // rec[(s * K + k) * 64 + l].
 constexpr int kBWord = 1;  // word holding b (ints 4..5 = .x, .y)
 
-// record prefetch distance in steps for K words per lane (registers:
-// D * 4K); a wave's step count is padded to a multiple of it
+// record prefetch distance D in steps for K words per lane; the records
+// of D + 1 steps are held in registers (4K each), and a wave's step count
+// is padded to a multiple of D + 1.  Loads and stores in flight stay below
+// the 63 the vector memory counter can track: D * (K + 2) < 63.
 constexpr int prefetch_steps(int K) {
-    return K <= 2 ? 16 : (K <= 3 ? 12 : (K <= 5 ? 8 : (K <= 6 ? 6 : (K <= 8 ? 5 : 4))));
+    return K <= 2 ? 14 : K <= 3 ? 12 : K <= 4 ? 10 : K <= 5 ? 8 : K <= 6 ? 6
+         : K <= 8 ? 5 : 4;
 }
+constexpr int block_steps(int K) { return prefetch_steps(K) + 1; }
 
 template <int NDPP, int NRING>
 struct Fmt {
@@ -85,6 +93,7 @@ struct Fmt {
     static constexpr int kInts = kVal + 2 * NRING;
     static constexpr int K = (kInts + 3) / 4;
     static constexpr int D = prefetch_steps(K);
+    static constexpr int NB = D + 1;  // record buffers, steps per block
 };
 
 __device__ inline int lds_load(const int *p) {
@@ -164,18 +173,32 @@ struct Shared {
     int abort;
 };
 
-// Consumer wave.  The step body is instantiated once per position U of a
-// block of D steps, so every pipeline register index is a compile-time
-// constant and the record prefetch stays in registers.  A wave's step count
-// is a multiple of D, so blocks have no exit: the loop latch then sees one
-// register assignment of the prefetch buffers, and no copies of loads in
-// flight (which would wait for them) are needed there.
+// Consumer wave.  The records of the next D steps are in flight into a
+// ring of D + 1 register buffers: step s first issues the load of step
+// s + D into the buffer step s - 1 has freed, then computes with its own.
+// The load then never waits for this step's chain or store, which a
+// wave's in-order issue would otherwise put in front of it.  The step body
+// is instantiated once per position U of a block of D + 1 steps, so every
+// buffer index is a compile-time constant and the ring stays in registers;
+// a wave's step count is a multiple of D + 1, so blocks have no exit and
+// the loop latch sees one register assignment of the ring.
+// BLK: a step has no branch (a branch costs a wave about as much as the
+// whole dependency chain of a step, taken or not): the FIFO is checked
+// once per block, for every group the block reads.  That is only safe when
+// no block waits, directly or through other blocks, for a row of a later
+// step of its own block -- the host checks the block wait graph for cycles
+// and otherwise builds with BLK off, which checks the FIFO at every step.
+// Each lane's record holds its group's FIFO offset from the block's first
+// group; consumption is reported once per block.
 // SEPB: b comes from a separate slot-order array (several right-hand
 // sides per launch share the records) instead of the record's b word.
-template <int NDPP, int NRING, bool SEPB>
+template <int NDPP, int NRING, bool SEPB, bool BLK>
 struct Consumer {
     using F = Fmt<NDPP, NRING>;
-    static constexpr int K = F::K, D = F::D;
+    static constexpr int K = F::K, D = F::D, NB = F::NB;
+    // a block's groups plus one resolver round fit the FIFO, so the
+    // resolver never waits for space the consumer is waiting for
+    static_assert(NB * 64 + 64 * kU <= kFF, "FIFO too small for a block");
 
     const FusedArgs &a;
     const int lane, ns, last;
@@ -185,9 +208,10 @@ struct Consumer {
     double *xring;
     const double *fifo;
     Shared &sh;
-    int4 rb[D][K];  // records of the next D steps
-    double bb[SEPB ? D : 1];  // b of the next D steps (SEPB)
-    int fifo_pos = 0, fifo_avail = 0, sig = 0;
+    int4 rb[NB][K];           // ring of step records
+    double bb[SEPB ? NB : 1];  // ring of b (SEPB)
+    int fifo_pos = 0, fifo_avail = 0, blk_end = 0;
+    double fv_next = 0.0;  // FIFO value of the next step (BLK)
     double xprev = 0.0;
     bool ok = true;
     unsigned long long t_wait = 0;
@@ -203,17 +227,15 @@ struct Consumer {
                   : nullptr),
           xring(xring_), fifo(fifo_), sh(sh_) {}
 
-    __device__ __forceinline__ void load_rec(int U, int t) {
+    __device__ __forceinline__ void load_rec(int B, int t) {
         const size_t tc = (size_t)min(t, last);
         const int4 *p = rec + tc * K * 64;
 #pragma unroll
-        for (int k = 0; k < K; k++) rb[U][k] = p[k * 64];
-        if constexpr (SEPB) bb[U] = br[tc * 64];
+        for (int k = 0; k < K; k++) rb[B][k] = p[k * 64];
+        if constexpr (SEPB) bb[B] = br[tc * 64];
     }
 
-    // Records of the first block, oldest first: the loop's wait for block
-    // position U counts the loads issued after U's, and the entry state must
-    // give the same count as the loop itself.
+    // records of steps 0 .. D-1, oldest first
     template <int U>
     __device__ __forceinline__ void prime() {
         if constexpr (U < D) {
@@ -248,12 +270,11 @@ struct Consumer {
 
     template <int U>
     __device__ __forceinline__ void step(int sb) {
-        if constexpr (U < D) {
+        if constexpr (U < NB) {
             const int s = sb + U;
-            // progress counters are wave-uniform; keep them scalar
-            fifo_pos = __builtin_amdgcn_readfirstlane(fifo_pos);
-            fifo_avail = __builtin_amdgcn_readfirstlane(fifo_avail);
-            sig = __builtin_amdgcn_readfirstlane(sig);
+            // the record of step s + D goes into the buffer of step s - 1
+            load_rec((U + D) % NB, s + D);
+            __builtin_amdgcn_sched_barrier(0);
             int f[4 * K];
 #pragma unroll
             for (int k = 0; k < K; k++) {
@@ -267,22 +288,39 @@ struct Consumer {
             double bv;
             if constexpr (SEPB) bv = bb[U];
             else bv = dbl(f[4], f[5]);
-            // everything that does not depend on the previous step's x
-            // first, so the chain that carries it is DPP + 3 FMAs + 2 ops
-            const int gtot = __builtin_amdgcn_readlane(f[1], 63) +
-                             (__builtin_amdgcn_readlane(flags, 63) & kHasGlobal);
-            if (gtot > 0) {
-                if (fifo_avail < fifo_pos + gtot) wait_fifo(fifo_pos + gtot);
-                const double fv = fifo[(fifo_pos + f[1]) & (kFF - 1)];
-                bv -= (flags & kHasGlobal) ? fv : 0.0;
-                fifo_pos += gtot;
+            if constexpr (U == 0) {
+                // the counters are wave-uniform: keep them scalar
+                fifo_pos = __builtin_amdgcn_readfirstlane(fifo_pos);
+                fifo_avail = __builtin_amdgcn_readfirstlane(fifo_avail);
+                blk_end = fifo_pos +
+                          (__builtin_amdgcn_readfirstlane(flags) >> kBlockShift);
+                // every group of the block must be in the FIFO
+                if constexpr (BLK)
+                    if (__builtin_expect(fifo_avail < blk_end, 0)) wait_fifo(blk_end);
             }
+            if constexpr (!BLK) {
+                // the groups of this step and the block's earlier steps
+                const int need = fifo_pos + __builtin_amdgcn_readlane(f[1], 63) +
+                                 (__builtin_amdgcn_readlane(flags, 63) & kHasGlobal);
+                if (__builtin_expect(fifo_avail < need, 0)) wait_fifo(need);
+            }
+            // FIFO values are read without a branch (lanes without a group
+            // discard them).  With BLK every group of the block is present
+            // once the block has started, so the next step's value is read
+            // one step early and the LDS latency leaves the chain.
+            double fv;
+            if constexpr (BLK && U > 0) fv = fv_next;
+            else fv = fifo[(fifo_pos + f[1]) & (kFF - 1)];
+            if constexpr (BLK && U + 1 < NB)
+                fv_next = fifo[(fifo_pos + rb[U + 1][0].y) & (kFF - 1)];
 #pragma unroll
             for (int j = 0; j < NRING; j++) {
                 const double xv =
                     *(const double *)((const char *)xring + f[F::kSrc + j]);
                 bv -= dbl(f[F::kVal + 2 * j], f[F::kVal + 2 * j + 1]) * xv;
             }
+            // the chain that carries the previous step's x: DPP + 3 FMAs,
+            // the subtractions and 1/diag
             double acc = 0.0;
             if constexpr (NDPP > 0) {
                 const double xl = dpp_from_lower(xprev);
@@ -291,21 +329,20 @@ struct Consumer {
                 acc += dbl(f[8], f[9]) * xl;
                 acc += dbl(f[10], f[11]) * xr;
             }
-            const double xv = (bv - acc) * invd;
+            const double t = bv - acc;
+            const double xv = ((flags & kHasGlobal) ? t - fv : t) * invd;
             xprev = xv;
             if constexpr (NRING > 0) xring[(s & (kR - 1)) * 64 + lane] = xv;
             // every slot goes out (the epilogue reads them); agent scope so
             // that published slots are visible to the other waves' resolvers
             st_x(&xq[(size_t)s * 64], xv);
-            if ((U == D - 1 && fifo_pos != sig) || fifo_pos - sig >= kSigLag) {
+            if constexpr (U == NB - 1) {
+                // the block's groups are consumed: their FIFO slots may be
+                // refilled
+                fifo_pos = blk_end;
                 wg_release();
                 if (lane == 0) lds_store(&sh.cons, fifo_pos);
-                sig = fifo_pos;
             }
-            // refill after the last use of this record: the new load can
-            // take the same registers
-            __builtin_amdgcn_sched_barrier(0);
-            load_rec(U, s + D);
             step<U + 1>(sb);
         }
     }
@@ -313,7 +350,7 @@ struct Consumer {
     __device__ __forceinline__ void run(int w, int r) {
         const unsigned long long t0 = a.trace ? now() : 0;
         prime<0>();
-        for (int sb = 0; sb < ns && ok; sb += D) step<0>(sb);
+        for (int sb = 0; sb < ns && ok; sb += NB) step<0>(sb);
         if (lane == 0) lds_store(&sh.cons, 1 << 30);
         if (a.trace && lane == 0 && r == 0) {
             unsigned long long *T = a.trace + (size_t)w * 16;
@@ -474,8 +511,8 @@ __device__ __forceinline__ void resolve(const FusedArgs &a, int w, int r,
 
 // One workgroup per (consumer wave w, right-hand side r): blockIdx.x =
 // r * W + w.  Right-hand sides share the records and nothing else.
-template <int NDPP, int NRING, bool SEPB>
-__global__ void __launch_bounds__(128) k_fused(FusedArgs a) {
+template <int NDPP, int NRING, bool SEPB, bool BLK>
+__global__ void __launch_bounds__(kWG) k_fused(FusedArgs a) {
     __shared__ double xring[NRING > 0 ? kR * 64 + 1 : 1];
     __shared__ double fifo[kFF];
     __shared__ double rtmp[64];
@@ -494,9 +531,9 @@ __global__ void __launch_bounds__(128) k_fused(FusedArgs a) {
     }
     __syncthreads();
     if (wave == 0) {
-        Consumer<NDPP, NRING, SEPB> c(a, w, r, lane, xring, fifo, sh);
+        Consumer<NDPP, NRING, SEPB, BLK> c(a, w, r, lane, xring, fifo, sh);
         c.run(w, r);
-    } else {
+    } else if (wave == 1) {
         resolve(a, w, r, lane, fifo, rtmp, gtmp, &ctmp, sh);
     }
 }
@@ -547,6 +584,7 @@ __global__ void k_fill_pending(double *p, long n) {
 
 struct FusedPlan {
     int W = 0, NDPP = 0, NRING = 0, K = 0, nsteps = 0, nent = 0, npub = 0;
+    bool blk = false;  // FIFO checked once per block (wait graph acyclic)
     unsigned epoch = 0;  // solves run; selects the slot-order x buffer
     int *d_step0 = nullptr, *d_gent0 = nullptr, *d_pub0 = nullptr;
     int4 *d_rec = nullptr;
@@ -577,6 +615,42 @@ static int words_for(int ndpp, int nring) {
 }
 
 static rb_status ensure_rhs(FusedPlan *F, int nr, hipStream_t s);
+
+// Can every block wait for all of its groups before its first step?  Block
+// X depends on the blocks holding the rows its global entries read and on
+// the previous block of its wave; the waits cannot close a cycle iff this
+// graph is acyclic (Kahn's algorithm over it).
+static bool wait_graph_acyclic(int nblk, const std::vector<int> &step0, int NB,
+                               const std::vector<std::pair<int, int>> &edges) {
+    std::vector<int> deps(nblk, 0), cnt(nblk + 1, 0);
+    std::vector<char> first(nblk, 0);
+    for (size_t w = 0; w + 1 < step0.size(); w++)
+        if (step0[w] < step0[w + 1]) first[step0[w] / NB] = 1;
+    for (const auto &e : edges) {
+        if (e.first == e.second) return false;
+        deps[e.first]++;
+        cnt[e.second + 1]++;
+    }
+    for (int b = 0; b < nblk; b++) {
+        if (!first[b]) {
+            deps[b]++;
+            cnt[b]++;  // b - 1 -> b
+        }
+    }
+    for (int b = 0; b < nblk; b++) cnt[b + 1] += cnt[b];
+    std::vector<int> rev(cnt[nblk]), fill(cnt.begin(), cnt.end() - 1);
+    for (const auto &e : edges) rev[fill[e.second]++] = e.first;
+    for (int b = 0; b < nblk; b++)
+        if (!first[b]) rev[fill[b - 1]++] = b;
+    std::vector<int> q;
+    q.reserve(nblk);
+    for (int b = 0; b < nblk; b++)
+        if (deps[b] == 0) q.push_back(b);
+    for (size_t h = 0; h < q.size(); h++)
+        for (int k = cnt[q[h]]; k < cnt[q[h] + 1]; k++)
+            if (--deps[rev[k]] == 0) q.push_back(rev[k]);
+    return (int)q.size() == nblk;
+}
 
 // Host partitioner and record builder.  rp/cc/cv: permuted CSR (ascending
 // columns); lp: BFS level boundaries; lf: final permutation; invd: 1/diag.
@@ -732,11 +806,11 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
     const int K = words_for(NDPP, NRING);
     // pad every wave to whole prefetch blocks with empty steps
     {
-        const int Dp = prefetch_steps(K);
+        const int Bp = block_steps(K);
         std::array<int, 64> empty;
         empty.fill(-1);
         for (int w = 0; w < W; w++)
-            while (steps[w].size() % Dp) steps[w].push_back(empty);
+            while (steps[w].size() % Bp) steps[w].push_back(empty);
     }
     const int kSrc = 6 + (NDPP ? 6 : 0), kVal = kSrc + ((NRING + 1) & ~1);
     // ---- emit records
@@ -758,13 +832,21 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
         f[at] = (int)(bits & 0xffffffff);
         f[at + 1] = (int)(bits >> 32);
     };
+    const int NB = block_steps(K);
+    // (waiting block, producing block) of every global entry; waves start
+    // at multiples of NB, so block = global step / NB
+    std::vector<std::pair<int, int>> wait_edges;
     for (int w = 0; w < W; w++) {
         gent0[w] = (int)gsrc.size();
-        int gord = 0;
+        int gord = 0, gofs = 0;
+        size_t bfirst = 0;
         for (int si = 0; si < (int)steps[w].size(); si++) {
             const auto &st = steps[w][si];
             const size_t sbase = (size_t)(step0[w] + si) * K * 64;
-            int gofs = 0;
+            if (si % NB == 0) {
+                bfirst = sbase;
+                gofs = 0;  // offsets restart at every block
+            }
             for (int l = 0; l < 64; l++) {
                 const int q = st[l];
                 std::fill(f.begin(), f.end(), 0);
@@ -787,6 +869,8 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
                             nl++;
                         } else {
                             needpub[u] = 1;
+                            wait_edges.emplace_back((step0[w] + si) / NB,
+                                                    slot_of(u) / 64 / NB);
                             gsrc.push_back(slot_of(u));
                             gval.push_back(cv[e]);
                             gmeta.push_back(gord * 2);
@@ -807,9 +891,12 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
                     rec[sbase + (size_t)k * 64 + l] =
                         make_int4(f[4 * k], f[4 * k + 1], f[4 * k + 2], f[4 * k + 3]);
             }
+            // the block's group count goes to lane 0 of its first step
+            if (si % NB == NB - 1) rec[bfirst].x |= gofs << kBlockShift;
         }
     }
     gent0[W] = (int)gsrc.size();
+    const bool blk = wait_graph_acyclic(NS / NB, step0, NB, wait_edges);
     // rows read by other waves (or beyond the ring window) publish their x
     std::vector<std::vector<int>> pubw(W);
     for (int q = 0; q < m; q++)
@@ -835,6 +922,7 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
     F->NDPP = NDPP;
     F->NRING = NRING;
     F->K = K;
+    F->blk = blk;
     F->nsteps = NS;
     F->nent = gent0[W];
     F->npub = pub0[W];
@@ -850,6 +938,10 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
     RB_TRY(A.alloc(&F->d_gmeta, gmeta.size()));
     RB_TRY(A.alloc(&F->d_pub, pub.size()));
     if (getenv("RB_FUSED_TRACE")) {
+        fprintf(stderr,
+                "fused schedule: %d waves, %d steps, %d words/lane, dpp %d, "
+                "ring %d, global entries %d, block-level FIFO checks %d\n",
+                W, NS, K, NDPP, NRING, gent0[W], (int)blk);
         RB_TRY(A.alloc(&F->d_trace, (size_t)W * 16));
         RB_HIP(hipMemsetAsync(F->d_trace, 0, sizeof(unsigned long long) * W * 16, s));
     }
@@ -913,18 +1005,20 @@ int fused_max_waves() {
 }
 
 template <int NDPP, int NRING, bool SEPB>
-static const void *kernel_of() {
-    return (const void *)k_fused<NDPP, NRING, SEPB>;
+static const void *kernel_of(bool blk) {
+    return blk ? (const void *)k_fused<NDPP, NRING, SEPB, true>
+               : (const void *)k_fused<NDPP, NRING, SEPB, false>;
 }
 
 template <bool SEPB>
 static const void *pick_kernel(const FusedPlan *F) {
     const int r = F->NRING;
+    const bool b = F->blk;
     if (F->NDPP)
-        return r == 0 ? kernel_of<3, 0, SEPB>() : r == 2 ? kernel_of<3, 2, SEPB>()
-             : r == 4 ? kernel_of<3, 4, SEPB>() : kernel_of<3, 8, SEPB>();
-    return r == 0 ? kernel_of<0, 0, SEPB>() : r == 2 ? kernel_of<0, 2, SEPB>()
-         : r == 4 ? kernel_of<0, 4, SEPB>() : kernel_of<0, 8, SEPB>();
+        return r == 0 ? kernel_of<3, 0, SEPB>(b) : r == 2 ? kernel_of<3, 2, SEPB>(b)
+             : r == 4 ? kernel_of<3, 4, SEPB>(b) : kernel_of<3, 8, SEPB>(b);
+    return r == 0 ? kernel_of<0, 0, SEPB>(b) : r == 2 ? kernel_of<0, 2, SEPB>(b)
+         : r == 4 ? kernel_of<0, 4, SEPB>(b) : kernel_of<0, 8, SEPB>(b);
 }
 
 // Slot-order x buffers for nr right-hand sides (grown on demand; a fresh
@@ -960,7 +1054,7 @@ static int rhs_per_launch(FusedPlan *F) {
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, pick_kernel<true>(F),
-                                                     128, 0) != hipSuccess)
+                                                     kWG, 0) != hipSuccess)
         per = 1;
     (void)hipGetLastError();
     F->nr_cap = std::max(1, std::min(64, (per * ncu) / std::max(1, F->W)));
@@ -1007,7 +1101,7 @@ rb_status solve_fused(rb_plan *p, const double *d_b, int ldb, double *d_x,
         mark_start(p, s);
         hipError_t e = hipLaunchCooperativeKernel(
             nr == 1 ? pick_kernel<false>(F) : pick_kernel<true>(F),
-            dim3(F->W * nr), dim3(128), kargs, 0, s);
+            dim3(F->W * nr), dim3(kWG), kargs, 0, s);
         if (e != hipSuccess)
             return status_of_hip(e, "hipLaunchCooperativeKernel(k_fused)");
         mark_end(p, s);
