@@ -20,7 +20,7 @@
 // A solve is three launches:
 //   prologue  writes b[orig] of every slot into the slot's step record;
 //   k_fused   one workgroup per consumer wave, plus a resolver wave:
-//     consumer  streams its step records (flags, 1/diag, b, coefficients,
+//     consumer  streams its step records (FIFO slot, 1/diag, b, coefficients,
 //               ring sources) into registers several steps ahead; per step
 //               x = (b - fifo - ring - dpp) / diag, written to the LDS ring
 //               and, contiguously, to the slot-order x array;
@@ -57,17 +57,26 @@ constexpr unsigned kSpin = 1u << 22;
 constexpr int kZeroOff = kR * 64 * 8;  // byte offset of the ring's zero
 constexpr int kWG = 128;        // consumer wave + resolver wave
 
-// record flags (int 0 of a lane's record)
-constexpr int kHasGlobal = 1;      // the row has a global group
-// bits 1.. of lane 0's flags in the first step of a block: the block's
-// group count
-constexpr int kBlockShift = 1;
-// int 1: the FIFO offset of the lane's group from the block's first group
-// (groups of the block's earlier steps and of lower lanes in this step)
+// FIFO slot of a lane's group, without a select: with p the block's first
+// FIFO byte (below kFFBytes) and g = int 1, the slot's byte address is
+//   ((p + g) & sel) | (g & ~sel),   sel = int 0
+// (one bit-field insert).  A lane with a group has sel = kFFBytes - 1 and
+// g = 8 * its group's offset from the block's first group (groups of the
+// block's earlier steps and of lower lanes in this step); a lane without
+// one has sel = 0 and g = kFFBytes, the zero slot behind the FIFO.  p + g
+// stays below 2 * kFFBytes, so bits kBlockShift.. of sel select zeros: they
+// hold group counts -- in lane 0 of a block's first step, the block's; in
+// lane 63 of every step, the block's groups up to and including the step.
+constexpr int kFFBytes = kFF * 8;
+constexpr int kHasGroup = kFFBytes - 1;  // sel of a lane with a group
+constexpr int kNoGroup = kFFBytes;       // g of a lane without one
+constexpr int kBlockShift = 15;
+static_assert(2 * kFFBytes <= (1 << kBlockShift), "block count bits");
 
 // Per-lane record of one step, as 32-bit words; every double starts at an
 // even word so it is an aligned register pair of the load that brings it:
-//   [0] flags  [1] block group offset  [2..3] 1/diag  [4..5] b (per solve)
+//   [0] sel (+ block group count)  [1] FIFO offset  [2..3] 1/diag
+//   [4..5] b (per solve)
 //   NDPP: [6..7] coefficient of lane l (previous step), [8..9] of lane l-1,
 //         [10..11] of lane l+1
 //   NRING ring sources (byte offsets into the ring), padded to even, then
@@ -129,6 +138,18 @@ __device__ inline int status_now(const int *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ inline double dbl(int lo, int hi) { return __hiloint2double(hi, lo); }
+
+// Buffer resources: a wave-uniform base and a size in bytes.  A buffer
+// access adds a scalar offset and a 32-bit lane offset in the memory unit
+// (no 64-bit address arithmetic in vector registers), and one outside the
+// size reads zero / is dropped.
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+constexpr int kBufWord3 = 0x00020000;  // raw 32-bit buffer (CDNA)
+constexpr int kSC1 = 16;               // cache policy: device-coherent (sc1)
+__device__ inline __amdgpu_buffer_rsrc_t buf_rsrc(const void *p, long bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<void *>(p), (short)0, (int)min(bytes, 0x7fffffffL), kBufWord3);
+}
 
 struct FusedArgs {
     const int *step0;    // [W+1] first step of each wave
@@ -202,15 +223,16 @@ struct Consumer {
 
     const FusedArgs &a;
     const int lane, ns, last;
-    const int4 *rec;
-    double *xq;
-    const double *br;
+    const int lane16, lane8;  // byte offsets of the lane's record / x
+    // this wave's records, slot-order x and b (SEPB)
+    const __amdgpu_buffer_rsrc_t rec, xq, br;
     double *xring;
     const double *fifo;
     Shared &sh;
     int4 rb[NB][K];           // ring of step records
     double bb[SEPB ? NB : 1];  // ring of b (SEPB)
     int fifo_pos = 0, fifo_avail = 0, blk_end = 0;
+    unsigned pos8 = 0;     // FIFO byte of the block's first group
     double fv_next = 0.0;  // FIFO value of the next step (BLK)
     double xprev = 0.0;
     bool ok = true;
@@ -221,18 +243,30 @@ struct Consumer {
                                         const double *fifo_, Shared &sh_)
         : a(a_), lane(lane_), ns(a_.step0[w + 1] - a_.step0[w]),
           last(a_.step0[w + 1] - a_.step0[w] - 1),
-          rec(a_.rec + (size_t)a_.step0[w] * K * 64 + lane_),
-          xq(a_.xq + r * a_.nslots + (size_t)a_.step0[w] * 64 + lane_),
-          br(SEPB ? a_.brec + r * a_.nslots + (size_t)a_.step0[w] * 64 + lane_
-                  : nullptr),
+          lane16(lane_ * 16), lane8(lane_ * 8),
+          rec(buf_rsrc(a_.rec + (size_t)a_.step0[w] * K * 64, (long)ns * K * 1024)),
+          xq(buf_rsrc(a_.xq + r * a_.nslots + (size_t)a_.step0[w] * 64, (long)ns * 512)),
+          br(buf_rsrc(SEPB ? a_.brec + r * a_.nslots + (size_t)a_.step0[w] * 64
+                           : a_.xq, SEPB ? (long)ns * 512 : 0)),
           xring(xring_), fifo(fifo_), sh(sh_) {}
 
+    // the FIFO slot of a lane's group, or the zero slot (record ints 0, 1)
+    __device__ __forceinline__ double fifo_at(int sel, int g) const {
+        const unsigned a = pos8 + (unsigned)g;
+        const unsigned o = (a & (unsigned)sel) | ((unsigned)g & ~(unsigned)sel);
+        return *(const double *)((const char *)fifo + o);
+    }
+
     __device__ __forceinline__ void load_rec(int B, int t) {
-        const size_t tc = (size_t)min(t, last);
-        const int4 *p = rec + tc * K * 64;
+        const int tc = min(t, last);
 #pragma unroll
-        for (int k = 0; k < K; k++) rb[B][k] = p[k * 64];
-        if constexpr (SEPB) bb[B] = br[tc * 64];
+        for (int k = 0; k < K; k++)
+            rb[B][k] = __builtin_bit_cast(
+                int4, __builtin_amdgcn_raw_buffer_load_b128(rec, lane16 + k * 1024,
+                                                             tc * K * 1024, 0));
+        if constexpr (SEPB)
+            bb[B] = __builtin_bit_cast(
+                double, __builtin_amdgcn_raw_buffer_load_b64(br, lane8, tc * 512, 0));
     }
 
     // records of steps 0 .. D-1, oldest first
@@ -283,7 +317,6 @@ struct Consumer {
                 f[4 * k + 2] = rb[U][k].z;
                 f[4 * k + 3] = rb[U][k].w;
             }
-            const int flags = f[0];
             const double invd = dbl(f[2], f[3]);
             double bv;
             if constexpr (SEPB) bv = bb[U];
@@ -292,27 +325,28 @@ struct Consumer {
                 // the counters are wave-uniform: keep them scalar
                 fifo_pos = __builtin_amdgcn_readfirstlane(fifo_pos);
                 fifo_avail = __builtin_amdgcn_readfirstlane(fifo_avail);
+                pos8 = (unsigned)(fifo_pos & (kFF - 1)) * 8;
                 blk_end = fifo_pos +
-                          (__builtin_amdgcn_readfirstlane(flags) >> kBlockShift);
+                          (__builtin_amdgcn_readfirstlane(f[0]) >> kBlockShift);
                 // every group of the block must be in the FIFO
                 if constexpr (BLK)
                     if (__builtin_expect(fifo_avail < blk_end, 0)) wait_fifo(blk_end);
             }
             if constexpr (!BLK) {
                 // the groups of this step and the block's earlier steps
-                const int need = fifo_pos + __builtin_amdgcn_readlane(f[1], 63) +
-                                 (__builtin_amdgcn_readlane(flags, 63) & kHasGlobal);
+                const int need = fifo_pos + (__builtin_amdgcn_readlane(f[0], 63) >>
+                                             kBlockShift);
                 if (__builtin_expect(fifo_avail < need, 0)) wait_fifo(need);
             }
             // FIFO values are read without a branch (lanes without a group
-            // discard them).  With BLK every group of the block is present
-            // once the block has started, so the next step's value is read
-            // one step early and the LDS latency leaves the chain.
+            // read the zero slot).  With BLK every group of the block is
+            // present once the block has started, so the next step's value
+            // is read one step early and the LDS latency leaves the chain.
             double fv;
             if constexpr (BLK && U > 0) fv = fv_next;
-            else fv = fifo[(fifo_pos + f[1]) & (kFF - 1)];
+            else fv = fifo_at(f[0], f[1]);
             if constexpr (BLK && U + 1 < NB)
-                fv_next = fifo[(fifo_pos + rb[U + 1][0].y) & (kFF - 1)];
+                fv_next = fifo_at(rb[U + 1][0].x, rb[U + 1][0].y);
 #pragma unroll
             for (int j = 0; j < NRING; j++) {
                 const double xv =
@@ -329,13 +363,14 @@ struct Consumer {
                 acc += dbl(f[8], f[9]) * xl;
                 acc += dbl(f[10], f[11]) * xr;
             }
-            const double t = bv - acc;
-            const double xv = ((flags & kHasGlobal) ? t - fv : t) * invd;
+            const double xv = ((bv - fv) - acc) * invd;
             xprev = xv;
             if constexpr (NRING > 0) xring[(s & (kR - 1)) * 64 + lane] = xv;
-            // every slot goes out (the epilogue reads them); agent scope so
-            // that published slots are visible to the other waves' resolvers
-            st_x(&xq[(size_t)s * 64], xv);
+            // every slot goes out (the epilogue reads them); device-coherent
+            // (as an agent-scope store) so that published slots are visible
+            // to the other waves' resolvers
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, xv), xq,
+                                                  lane8, s * 512, kSC1);
             if constexpr (U == NB - 1) {
                 // the block's groups are consumed: their FIFO slots may be
                 // refilled
@@ -514,7 +549,7 @@ __device__ __forceinline__ void resolve(const FusedArgs &a, int w, int r,
 template <int NDPP, int NRING, bool SEPB, bool BLK>
 __global__ void __launch_bounds__(kWG) k_fused(FusedArgs a) {
     __shared__ double xring[NRING > 0 ? kR * 64 + 1 : 1];
-    __shared__ double fifo[kFF];
+    __shared__ double fifo[kFF + 1];  // + the zero slot
     __shared__ double rtmp[64];
     __shared__ int gtmp[64];
     __shared__ double ctmp;
@@ -528,6 +563,7 @@ __global__ void __launch_bounds__(kWG) k_fused(FusedArgs a) {
         sh.cons = 0;
         sh.abort = 0;
         if (NRING > 0) xring[kR * 64] = 0.0;
+        fifo[kFF] = 0.0;
     }
     __syncthreads();
     if (wave == 0) {
@@ -851,12 +887,11 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
                 const int q = st[l];
                 std::fill(f.begin(), f.end(), 0);
                 for (int j = 0; j < NRING; j++) f[kSrc + j] = kZeroOff;
-                int flags = 0;
+                bool hasg = false;
                 if (q >= 0) {
                     const int r = pof(q);
                     origv[(size_t)(step0[w] + si) * 64 + l] = lf[r];
                     int nl = 0;
-                    bool hasg = false;
                     for (int e = rp[r]; e < rp[r + 1]; e++) {
                         if (cc[e] == r) continue;
                         const int u = qof(cc[e]);
@@ -880,13 +915,13 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
                     if (hasg) {
                         gmeta.back() |= 1;
                         gord++;
-                        flags = kHasGlobal;
                     }
                     put(2, invd[r]);
                 }
-                f[0] = flags;
-                f[1] = gofs;
-                gofs += flags & kHasGlobal;
+                f[0] = hasg ? kHasGroup : 0;
+                f[1] = hasg ? gofs * 8 : kNoGroup;
+                gofs += hasg;
+                if (l == 63) f[0] |= gofs << kBlockShift;
                 for (int k = 0; k < K; k++)
                     rec[sbase + (size_t)k * 64 + l] =
                         make_int4(f[4 * k], f[4 * k + 1], f[4 * k + 2], f[4 * k + 3]);
