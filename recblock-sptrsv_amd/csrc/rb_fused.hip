@@ -75,15 +75,20 @@ static_assert(2 * kFFBytes <= (1 << kBlockShift), "block count bits");
 
 // Per-lane record of one step, as 32-bit words; every double starts at an
 // even word so it is an aligned register pair of the load that brings it:
-//   [0] sel (+ block group count)  [1] FIFO offset  [2..3] 1/diag
-//   [4..5] b (per solve)
-//   NDPP: [6..7] coefficient of lane l (previous step), [8..9] of lane l-1,
-//         [10..11] of lane l+1
+//   [0] sel (+ group counts)  [1] FIFO offset
+//   NDPP: [2..3] coefficient of lane l (previous step), [4..5] of lane l-1,
+//         [6..7] of lane l+1
 //   NRING ring sources (byte offsets into the ring), padded to even, then
 //   NRING coefficients
 // padded to whole 16-byte words; word k of step s for lane l lives at
-// rec[(s * K + k) * 64 + l].
-constexpr int kBWord = 1;  // word holding b (ints 4..5 = .x, .y)
+// rec[(s * K + k) * 64 + l].  Every word is read: a loaded word the step
+// does not use would free its registers while the load is in flight.
+// Every coefficient is stored as -a_ij / a_ii, and the global entries'
+// values as a_ij / a_ii, so a row is x_i = b_i/a_ii + sum(coef * x_j) - fifo:
+// fused multiply-adds only, without a subtraction or a product by 1/diag
+// on the chain from the previous step's x.  b_i/a_ii comes per solve from a
+// separate slot-order array (one per right-hand side), written by the
+// prologue.
 
 // record prefetch distance D in steps for K words per lane; the records
 // of D + 1 steps are held in registers (4K each), and a wave's step count
@@ -97,7 +102,7 @@ constexpr int block_steps(int K) { return prefetch_steps(K) + 1; }
 
 template <int NDPP, int NRING>
 struct Fmt {
-    static constexpr int kSrc = 6 + (NDPP ? 6 : 0);
+    static constexpr int kSrc = 2 + (NDPP ? 6 : 0);
     static constexpr int kVal = kSrc + ((NRING + 1) & ~1);
     static constexpr int kInts = kVal + 2 * NRING;
     static constexpr int K = (kInts + 3) / 4;
@@ -162,7 +167,7 @@ struct FusedArgs {
     const int *pub;      // [npub] published slots, grouped by wave
     double *xq;          // [nr][nsteps * 64] x in slot order (this solve)
     double *xq_next;     // the other buffers, re-armed for the next solve
-    const double *brec;  // [nr][nsteps * 64] b in slot order (SEPB)
+    const double *brec;  // [nr][nsteps * 64] b/diag in slot order
     long nslots;         // nsteps * 64: stride between right-hand sides
     int W;               // consumer waves per right-hand side
     int *status;
@@ -211,9 +216,7 @@ struct Shared {
 // and otherwise builds with BLK off, which checks the FIFO at every step.
 // Each lane's record holds its group's FIFO offset from the block's first
 // group; consumption is reported once per block.
-// SEPB: b comes from a separate slot-order array (several right-hand
-// sides per launch share the records) instead of the record's b word.
-template <int NDPP, int NRING, bool SEPB, bool BLK>
+template <int NDPP, int NRING, bool BLK>
 struct Consumer {
     using F = Fmt<NDPP, NRING>;
     static constexpr int K = F::K, D = F::D, NB = F::NB;
@@ -224,13 +227,13 @@ struct Consumer {
     const FusedArgs &a;
     const int lane, ns, last;
     const int lane16, lane8;  // byte offsets of the lane's record / x
-    // this wave's records, slot-order x and b (SEPB)
+    // this wave's records, slot-order x and b/diag
     const __amdgpu_buffer_rsrc_t rec, xq, br;
     double *xring;
     const double *fifo;
     Shared &sh;
     int4 rb[NB][K];           // ring of step records
-    double bb[SEPB ? NB : 1];  // ring of b (SEPB)
+    double bb[NB];            // ring of b/diag
     int fifo_pos = 0, fifo_avail = 0, blk_end = 0;
     unsigned pos8 = 0;     // FIFO byte of the block's first group
     double fv_next = 0.0;  // FIFO value of the next step (BLK)
@@ -246,8 +249,8 @@ struct Consumer {
           lane16(lane_ * 16), lane8(lane_ * 8),
           rec(buf_rsrc(a_.rec + (size_t)a_.step0[w] * K * 64, (long)ns * K * 1024)),
           xq(buf_rsrc(a_.xq + r * a_.nslots + (size_t)a_.step0[w] * 64, (long)ns * 512)),
-          br(buf_rsrc(SEPB ? a_.brec + r * a_.nslots + (size_t)a_.step0[w] * 64
-                           : a_.xq, SEPB ? (long)ns * 512 : 0)),
+          br(buf_rsrc(a_.brec + r * a_.nslots + (size_t)a_.step0[w] * 64,
+                      (long)ns * 512)),
           xring(xring_), fifo(fifo_), sh(sh_) {}
 
     // the FIFO slot of a lane's group, or the zero slot (record ints 0, 1)
@@ -264,8 +267,7 @@ struct Consumer {
             rb[B][k] = __builtin_bit_cast(
                 int4, __builtin_amdgcn_raw_buffer_load_b128(rec, lane16 + k * 1024,
                                                              tc * K * 1024, 0));
-        if constexpr (SEPB)
-            bb[B] = __builtin_bit_cast(
+        bb[B] = __builtin_bit_cast(
                 double, __builtin_amdgcn_raw_buffer_load_b64(br, lane8, tc * 512, 0));
     }
 
@@ -317,10 +319,7 @@ struct Consumer {
                 f[4 * k + 2] = rb[U][k].z;
                 f[4 * k + 3] = rb[U][k].w;
             }
-            const double invd = dbl(f[2], f[3]);
-            double bv;
-            if constexpr (SEPB) bv = bb[U];
-            else bv = dbl(f[4], f[5]);
+            double bv = bb[U];
             if constexpr (U == 0) {
                 // the counters are wave-uniform: keep them scalar
                 fifo_pos = __builtin_amdgcn_readfirstlane(fifo_pos);
@@ -345,25 +344,33 @@ struct Consumer {
             double fv;
             if constexpr (BLK && U > 0) fv = fv_next;
             else fv = fifo_at(f[0], f[1]);
-            if constexpr (BLK && U + 1 < NB)
+            if constexpr (BLK && U + 1 < NB) {
                 fv_next = fifo_at(rb[U + 1][0].x, rb[U + 1][0].y);
+                // issue the read here, not behind this step's chain
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            // with BLK the FIFO value is at hand (read a step early) and
+            // goes in first; otherwise it is subtracted last, so the wait
+            // for its LDS read sits behind the chain
+            if constexpr (BLK) bv -= fv;
 #pragma unroll
             for (int j = 0; j < NRING; j++) {
                 const double xv =
                     *(const double *)((const char *)xring + f[F::kSrc + j]);
-                bv -= dbl(f[F::kVal + 2 * j], f[F::kVal + 2 * j + 1]) * xv;
+                bv = __builtin_fma(dbl(f[F::kVal + 2 * j], f[F::kVal + 2 * j + 1]),
+                                   xv, bv);
             }
-            // the chain that carries the previous step's x: DPP + 3 FMAs,
-            // the subtractions and 1/diag
-            double acc = 0.0;
+            // the chain that carries the previous step's x: 3 FMAs (the
+            // first needs no DPP)
+            double acc = bv;
             if constexpr (NDPP > 0) {
                 const double xl = dpp_from_lower(xprev);
                 const double xr = dpp_from_upper(xprev);
-                acc = dbl(f[6], f[7]) * xprev;
-                acc += dbl(f[8], f[9]) * xl;
-                acc += dbl(f[10], f[11]) * xr;
+                acc = __builtin_fma(dbl(f[2], f[3]), xprev, acc);
+                acc = __builtin_fma(dbl(f[4], f[5]), xl, acc);
+                acc = __builtin_fma(dbl(f[6], f[7]), xr, acc);
             }
-            const double xv = ((bv - fv) - acc) * invd;
+            const double xv = BLK ? acc : acc - fv;
             xprev = xv;
             if constexpr (NRING > 0) xring[(s & (kR - 1)) * 64 + lane] = xv;
             // every slot goes out (the epilogue reads them); device-coherent
@@ -546,7 +553,7 @@ __device__ __forceinline__ void resolve(const FusedArgs &a, int w, int r,
 
 // One workgroup per (consumer wave w, right-hand side r): blockIdx.x =
 // r * W + w.  Right-hand sides share the records and nothing else.
-template <int NDPP, int NRING, bool SEPB, bool BLK>
+template <int NDPP, int NRING, bool BLK>
 __global__ void __launch_bounds__(kWG) k_fused(FusedArgs a) {
     __shared__ double xring[NRING > 0 ? kR * 64 + 1 : 1];
     __shared__ double fifo[kFF + 1];  // + the zero slot
@@ -567,35 +574,23 @@ __global__ void __launch_bounds__(kWG) k_fused(FusedArgs a) {
     }
     __syncthreads();
     if (wave == 0) {
-        Consumer<NDPP, NRING, SEPB, BLK> c(a, w, r, lane, xring, fifo, sh);
+        Consumer<NDPP, NRING, BLK> c(a, w, r, lane, xring, fifo, sh);
         c.run(w, r);
     } else if (wave == 1) {
         resolve(a, w, r, lane, fifo, rtmp, gtmp, &ctmp, sh);
     }
 }
 
-// b of every slot into its step record (0 for padding lanes)
-__global__ void k_fused_prologue(const int *__restrict__ orig, long nslots,
-                                 int K, const double *__restrict__ b,
-                                 int4 *rec) {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nslots) return;
-    const int o = orig[i];
-    const double v = o >= 0 ? b[o] : 0.0;
-    int2 *p = (int2 *)&rec[((i >> 6) * K + kBWord) * 64 + (i & 63)];
-    *p = make_int2(__double2loint(v), __double2hiint(v));
-}
-
-// b of nr right-hand sides in slot order (0 for padding lanes)
-__global__ void k_fused_prologue_multi(const int *__restrict__ orig,
-                                       long nslots, int nr,
-                                       const double *__restrict__ b, long ldb,
-                                       double *brec) {
+// b/diag of nr right-hand sides in slot order (0 for padding lanes)
+__global__ void k_fused_prologue(const int *__restrict__ orig,
+                                 const double *__restrict__ sinvd, long nslots,
+                                 int nr, const double *__restrict__ b, long ldb,
+                                 double *brec) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nslots * nr) return;
     const long r = i / nslots, q = i - r * nslots;
     const int o = orig[q];
-    brec[i] = o >= 0 ? b[r * ldb + o] : 0.0;
+    brec[i] = o >= 0 ? b[r * ldb + o] * sinvd[q] : 0.0;
 }
 
 // slot-order x of nr right-hand sides to the caller's x
@@ -633,7 +628,8 @@ struct FusedPlan {
     double *d_xq = nullptr;
     int nr_alloc = 0;
     std::vector<char> armed[2];
-    double *d_brec = nullptr;  // b in slot order, nr_alloc right-hand sides
+    double *d_brec = nullptr;  // b/diag in slot order, nr_alloc right-hand sides
+    double *d_sinvd = nullptr; // 1/diag in slot order (0 for padding lanes)
     int nr_cap = 0;            // right-hand sides per launch (co-residency)
     unsigned long long *d_trace = nullptr;  // RB_FUSED_TRACE=1
     double build_ms = 0.0;
@@ -646,7 +642,7 @@ struct FusedPlan {
 static int ring_class(int e) { return e == 0 ? 0 : (e <= 2 ? 2 : (e <= 4 ? 4 : 8)); }
 
 static int words_for(int ndpp, int nring) {
-    const int ints = 6 + (ndpp ? 6 : 0) + ((nring + 1) & ~1) + 2 * nring;
+    const int ints = 2 + (ndpp ? 6 : 0) + ((nring + 1) & ~1) + 2 * nring;
     return (ints + 3) / 4;
 }
 
@@ -848,7 +844,7 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
         for (int w = 0; w < W; w++)
             while (steps[w].size() % Bp) steps[w].push_back(empty);
     }
-    const int kSrc = 6 + (NDPP ? 6 : 0), kVal = kSrc + ((NRING + 1) & ~1);
+    const int kSrc = 2 + (NDPP ? 6 : 0), kVal = kSrc + ((NRING + 1) & ~1);
     // ---- emit records
     std::vector<int> step0(W + 1, 0), gent0(W + 1, 0), pub0(W + 1, 0);
     for (int w = 0; w < W; w++) step0[w + 1] = step0[w] + (int)steps[w].size();
@@ -858,6 +854,7 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
     };
     std::vector<int4> rec((size_t)NS * K * 64);
     std::vector<int> origv((size_t)NS * 64, -1);
+    std::vector<double> sinvd((size_t)NS * 64, 0.0);
     std::vector<int> gsrc, gmeta;
     std::vector<double> gval;
     std::vector<char> needpub(m, 0);
@@ -897,17 +894,17 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
                         const int u = qof(cc[e]);
                         if (NDPP && is_dpp(q, u)) {
                             const int du = lanew[u] - l;  // -1, 0, +1
-                            put(du == 0 ? 6 : (du < 0 ? 8 : 10), cv[e]);
+                            put(du == 0 ? 2 : (du < 0 ? 4 : 6), -cv[e] * invd[r]);
                         } else if (is_loc(q, u) && nl < NRING) {
                             f[kSrc + nl] = ((stepw[u] % kR) * 64 + lanew[u]) * 8;
-                            put(kVal + 2 * nl, cv[e]);
+                            put(kVal + 2 * nl, -cv[e] * invd[r]);
                             nl++;
                         } else {
                             needpub[u] = 1;
                             wait_edges.emplace_back((step0[w] + si) / NB,
                                                     slot_of(u) / 64 / NB);
                             gsrc.push_back(slot_of(u));
-                            gval.push_back(cv[e]);
+                            gval.push_back(cv[e] * invd[r]);
                             gmeta.push_back(gord * 2);
                             hasg = true;
                         }
@@ -916,7 +913,7 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
                         gmeta.back() |= 1;
                         gord++;
                     }
-                    put(2, invd[r]);
+                    sinvd[(size_t)(step0[w] + si) * 64 + l] = invd[r];
                 }
                 f[0] = hasg ? kHasGroup : 0;
                 f[1] = hasg ? gofs * 8 : kNoGroup;
@@ -968,6 +965,7 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
     RB_TRY(A.alloc(&F->d_pub0, W + 1));
     RB_TRY(A.alloc(&F->d_rec, rec.size()));
     RB_TRY(A.alloc(&F->d_orig, nslots));
+    RB_TRY(A.alloc(&F->d_sinvd, nslots));
     RB_TRY(A.alloc(&F->d_gsrc, gsrc.size()));
     RB_TRY(A.alloc(&F->d_gval, gval.size()));
     RB_TRY(A.alloc(&F->d_gmeta, gmeta.size()));
@@ -989,6 +987,7 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
     RB_TRY(up(F->d_pub0, pub0.data(), sizeof(int) * pub0.size()));
     RB_TRY(up(F->d_rec, rec.data(), sizeof(int4) * rec.size()));
     RB_TRY(up(F->d_orig, origv.data(), sizeof(int) * origv.size()));
+    RB_TRY(up(F->d_sinvd, sinvd.data(), sizeof(double) * sinvd.size()));
     RB_TRY(up(F->d_gsrc, gsrc.data(), sizeof(int) * gsrc.size()));
     RB_TRY(up(F->d_gval, gval.data(), sizeof(double) * gval.size()));
     RB_TRY(up(F->d_gmeta, gmeta.data(), sizeof(int) * gmeta.size()));
@@ -1039,21 +1038,20 @@ int fused_max_waves() {
     return ncu > 0 ? ncu : 256;
 }
 
-template <int NDPP, int NRING, bool SEPB>
+template <int NDPP, int NRING>
 static const void *kernel_of(bool blk) {
-    return blk ? (const void *)k_fused<NDPP, NRING, SEPB, true>
-               : (const void *)k_fused<NDPP, NRING, SEPB, false>;
+    return blk ? (const void *)k_fused<NDPP, NRING, true>
+               : (const void *)k_fused<NDPP, NRING, false>;
 }
 
-template <bool SEPB>
 static const void *pick_kernel(const FusedPlan *F) {
     const int r = F->NRING;
     const bool b = F->blk;
     if (F->NDPP)
-        return r == 0 ? kernel_of<3, 0, SEPB>(b) : r == 2 ? kernel_of<3, 2, SEPB>(b)
-             : r == 4 ? kernel_of<3, 4, SEPB>(b) : kernel_of<3, 8, SEPB>(b);
-    return r == 0 ? kernel_of<0, 0, SEPB>(b) : r == 2 ? kernel_of<0, 2, SEPB>(b)
-         : r == 4 ? kernel_of<0, 4, SEPB>(b) : kernel_of<0, 8, SEPB>(b);
+        return r == 0 ? kernel_of<3, 0>(b) : r == 2 ? kernel_of<3, 2>(b)
+             : r == 4 ? kernel_of<3, 4>(b) : kernel_of<3, 8>(b);
+    return r == 0 ? kernel_of<0, 0>(b) : r == 2 ? kernel_of<0, 2>(b)
+         : r == 4 ? kernel_of<0, 4>(b) : kernel_of<0, 8>(b);
 }
 
 // Slot-order x buffers for nr right-hand sides (grown on demand; a fresh
@@ -1068,7 +1066,7 @@ static rb_status ensure_rhs(FusedPlan *F, int nr, hipStream_t s) {
     F->d_brec = nullptr;
     F->nr_alloc = 0;
     if (hipMalloc(&F->d_xq, 2 * nr * nslots * sizeof(double)) != hipSuccess ||
-        (nr > 1 && hipMalloc(&F->d_brec, nr * nslots * sizeof(double)) != hipSuccess)) {
+        hipMalloc(&F->d_brec, nr * nslots * sizeof(double)) != hipSuccess) {
         (void)hipGetLastError();
         set_error(RB_ERR_NO_MEMORY, "hipMalloc failed (slot-order x buffers)");
         return RB_ERR_NO_MEMORY;
@@ -1088,7 +1086,7 @@ static int rhs_per_launch(FusedPlan *F) {
     int dev = 0, ncu = 0, per = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, pick_kernel<true>(F),
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, pick_kernel(F),
                                                      kWG, 0) != hipSuccess)
         per = 1;
     (void)hipGetLastError();
@@ -1122,12 +1120,8 @@ rb_status solve_fused(rb_plan *p, const double *d_b, int ldb, double *d_x,
         const double *b = d_b + (size_t)k0 * ldb;
         const long tot = (long)nslots * nr;
         const int grid = (int)((tot + 255) / 256);
-        if (nr == 1)
-            k_fused_prologue<<<grid, 256, 0, s>>>(F->d_orig, (long)nslots, F->K,
-                                                  b, F->d_rec);
-        else
-            k_fused_prologue_multi<<<grid, 256, 0, s>>>(F->d_orig, (long)nslots,
-                                                        nr, b, ldb, F->d_brec);
+        k_fused_prologue<<<grid, 256, 0, s>>>(F->d_orig, F->d_sinvd, (long)nslots,
+                                              nr, b, ldb, F->d_brec);
         RB_HIP(hipGetLastError());
         FusedArgs args{F->d_step0, F->d_gent0, F->d_pub0, F->d_rec,
                        F->d_gsrc, F->d_gval, F->d_gmeta, F->d_pub, cur, nxt,
@@ -1135,7 +1129,7 @@ rb_status solve_fused(rb_plan *p, const double *d_b, int ldb, double *d_x,
         void *kargs[] = {(void *)&args};
         mark_start(p, s);
         hipError_t e = hipLaunchCooperativeKernel(
-            nr == 1 ? pick_kernel<false>(F) : pick_kernel<true>(F),
+            pick_kernel(F),
             dim3(F->W * nr), dim3(kWG), kargs, 0, s);
         if (e != hipSuccess)
             return status_of_hip(e, "hipLaunchCooperativeKernel(k_fused)");
