@@ -51,27 +51,26 @@ namespace rb {
 namespace {
 
 constexpr int kR = 32;          // LDS x ring depth (steps), a power of two
-constexpr int kFF = 2048;       // FIFO capacity (groups), a power of two
 constexpr int kU = 2;           // resolver: batches of 64 polled together
 constexpr unsigned kSpin = 1u << 22;
 constexpr int kZeroOff = kR * 64 * 8;  // byte offset of the ring's zero
 constexpr int kWG = 128;        // consumer wave + resolver wave
 
 // FIFO slot of a lane's group, without a select: with p the block's first
-// FIFO byte (below kFFBytes) and g = int 1, the slot's byte address is
+// FIFO byte (below FF * 8 for a FIFO of FF groups) and g = int 1, the
+// slot's byte address is
 //   ((p + g) & sel) | (g & ~sel),   sel = int 0
-// (one bit-field insert).  A lane with a group has sel = kFFBytes - 1 and
+// (one bit-field insert).  A lane with a group has sel = FF * 8 - 1 and
 // g = 8 * its group's offset from the block's first group (groups of the
 // block's earlier steps and of lower lanes in this step); a lane without
-// one has sel = 0 and g = kFFBytes, the zero slot behind the FIFO.  p + g
-// stays below 2 * kFFBytes, so bits kBlockShift.. of sel select zeros: they
-// hold group counts -- in lane 0 of a block's first step, the block's; in
-// lane 63 of every step, the block's groups up to and including the step.
-constexpr int kFFBytes = kFF * 8;
-constexpr int kHasGroup = kFFBytes - 1;  // sel of a lane with a group
-constexpr int kNoGroup = kFFBytes;       // g of a lane without one
+// one has sel = 0 and g = FF * 8, the zero slot behind the FIFO.  p + g
+// stays below 2 * FF * 8 <= 2^kBlockShift, so bits kBlockShift.. of sel
+// select zeros: they hold group counts -- in lane 0 of a block's first
+// step, the block's; in lane 63 of every step, the block's groups up to
+// and including the step.
 constexpr int kBlockShift = 15;
-static_assert(2 * kFFBytes <= (1 << kBlockShift), "block count bits");
+constexpr int has_group_sel(int FF) { return FF * 8 - 1; }
+constexpr int no_group_off(int FF) { return FF * 8; }
 
 // Per-lane record of one step, as 32-bit words; every double starts at an
 // even word so it is an aligned register pair of the load that brings it:
@@ -93,12 +92,20 @@ static_assert(2 * kFFBytes <= (1 << kBlockShift), "block count bits");
 // record prefetch distance D in steps for K words per lane; the records
 // of D + 1 steps are held in registers (4K each), and a wave's step count
 // is padded to a multiple of D + 1.  Loads and stores in flight stay below
-// the 63 the vector memory counter can track: D * (K + 2) < 63.
+// the 63 the vector memory counter can track (D * (K + 2) < 63), and for
+// K > 2 the register ring stays near 140 registers so that three waves fit
+// a SIMD (6 workgroups per compute unit for several right-hand sides).
 constexpr int prefetch_steps(int K) {
-    return K <= 2 ? 14 : K <= 3 ? 12 : K <= 4 ? 10 : K <= 5 ? 8 : K <= 6 ? 6
-         : K <= 8 ? 5 : 4;
+    return K <= 2 ? 15 : K <= 3 ? 10 : K <= 4 ? 7 : K <= 5 ? 6 : K <= 7 ? 4
+         : K <= 8 ? 3 : 2;
 }
 constexpr int block_steps(int K) { return prefetch_steps(K) + 1; }
+// FIFO capacity in groups (a power of two): a block's groups (at most 64
+// per step) plus one resolver round must fit
+constexpr int fifo_groups(int K) {
+    return block_steps(K) * 64 + 64 * kU <= 1024 ? 1024 : 2048;
+}
+static_assert(2 * 2048 * 8 <= (1 << kBlockShift), "group count bits");
 
 template <int NDPP, int NRING>
 struct Fmt {
@@ -108,6 +115,7 @@ struct Fmt {
     static constexpr int K = (kInts + 3) / 4;
     static constexpr int D = prefetch_steps(K);
     static constexpr int NB = D + 1;  // record buffers, steps per block
+    static constexpr int FF = fifo_groups(K);
 };
 
 __device__ inline int lds_load(const int *p) {
@@ -219,10 +227,10 @@ struct Shared {
 template <int NDPP, int NRING, bool BLK>
 struct Consumer {
     using F = Fmt<NDPP, NRING>;
-    static constexpr int K = F::K, D = F::D, NB = F::NB;
+    static constexpr int K = F::K, D = F::D, NB = F::NB, FF = F::FF;
     // a block's groups plus one resolver round fit the FIFO, so the
     // resolver never waits for space the consumer is waiting for
-    static_assert(NB * 64 + 64 * kU <= kFF, "FIFO too small for a block");
+    static_assert(NB * 64 + 64 * kU <= FF, "FIFO too small for a block");
 
     const FusedArgs &a;
     const int lane, ns, last;
@@ -324,7 +332,7 @@ struct Consumer {
                 // the counters are wave-uniform: keep them scalar
                 fifo_pos = __builtin_amdgcn_readfirstlane(fifo_pos);
                 fifo_avail = __builtin_amdgcn_readfirstlane(fifo_avail);
-                pos8 = (unsigned)(fifo_pos & (kFF - 1)) * 8;
+                pos8 = (unsigned)(fifo_pos & (FF - 1)) * 8;
                 blk_end = fifo_pos +
                           (__builtin_amdgcn_readfirstlane(f[0]) >> kBlockShift);
                 // every group of the block must be in the FIFO
@@ -403,6 +411,7 @@ struct Consumer {
     }
 };
 
+template <int FF>
 __device__ __forceinline__ void resolve(const FusedArgs &a, int w, int r,
                                         int lane, double *fifo, double *rtmp,
                                         int *gtmp, double *ctmp, Shared &sh) {
@@ -443,7 +452,7 @@ __device__ __forceinline__ void resolve(const FusedArgs &a, int w, int r,
             const int gmax = a.gmeta[kb + nt - 1] >> 1;
             unsigned spins = 0;
 #pragma nounroll
-            while (gmax - lds_load_u(&sh.cons) >= kFF) {
+            while (gmax - lds_load_u(&sh.cons) >= FF) {
                 if (++spins > kSpin || lds_load_u(&sh.abort)) {
                     abort = true;
                     break;
@@ -504,7 +513,7 @@ __device__ __forceinline__ void resolve(const FusedArgs &a, int w, int r,
                             if (l + 1 >= hi || (gtmp[l + 1] >> 1) != g) break;
                             l++;
                         }
-                        if (gtmp[l] & 1) fifo[g & (kFF - 1)] = sum;  // complete
+                        if (gtmp[l] & 1) fifo[g & (FF - 1)] = sum;  // complete
                         else *ctmp = sum;  // only the batch's last group
                     }
                     const int gl = __shfl(gm, hi - 1, 64);
@@ -554,9 +563,10 @@ __device__ __forceinline__ void resolve(const FusedArgs &a, int w, int r,
 // One workgroup per (consumer wave w, right-hand side r): blockIdx.x =
 // r * W + w.  Right-hand sides share the records and nothing else.
 template <int NDPP, int NRING, bool BLK>
-__global__ void __launch_bounds__(kWG) k_fused(FusedArgs a) {
+__global__ void __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(BLK ? 3 : 1))) k_fused(FusedArgs a) {
     __shared__ double xring[NRING > 0 ? kR * 64 + 1 : 1];
-    __shared__ double fifo[kFF + 1];  // + the zero slot
+    constexpr int FF = Fmt<NDPP, NRING>::FF;
+    __shared__ double fifo[FF + 1];  // + the zero slot
     __shared__ double rtmp[64];
     __shared__ int gtmp[64];
     __shared__ double ctmp;
@@ -570,14 +580,14 @@ __global__ void __launch_bounds__(kWG) k_fused(FusedArgs a) {
         sh.cons = 0;
         sh.abort = 0;
         if (NRING > 0) xring[kR * 64] = 0.0;
-        fifo[kFF] = 0.0;
+        fifo[FF] = 0.0;
     }
     __syncthreads();
     if (wave == 0) {
         Consumer<NDPP, NRING, BLK> c(a, w, r, lane, xring, fifo, sh);
         c.run(w, r);
     } else if (wave == 1) {
-        resolve(a, w, r, lane, fifo, rtmp, gtmp, &ctmp, sh);
+        resolve<FF>(a, w, r, lane, fifo, rtmp, gtmp, &ctmp, sh);
     }
 }
 
@@ -865,7 +875,7 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
         f[at] = (int)(bits & 0xffffffff);
         f[at + 1] = (int)(bits >> 32);
     };
-    const int NB = block_steps(K);
+    const int NB = block_steps(K), FF = fifo_groups(K);
     // (waiting block, producing block) of every global entry; waves start
     // at multiples of NB, so block = global step / NB
     std::vector<std::pair<int, int>> wait_edges;
@@ -915,8 +925,8 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
                     }
                     sinvd[(size_t)(step0[w] + si) * 64 + l] = invd[r];
                 }
-                f[0] = hasg ? kHasGroup : 0;
-                f[1] = hasg ? gofs * 8 : kNoGroup;
+                f[0] = hasg ? has_group_sel(FF) : 0;
+                f[1] = hasg ? gofs * 8 : no_group_off(FF);
                 gofs += hasg;
                 if (l == 63) f[0] |= gofs << kBlockShift;
                 for (int k = 0; k < K; k++)
