@@ -261,6 +261,43 @@ rb_status rb_plan_kernel_ms(rb_plan *plan, double *ms);
 rb_status rb_plan_permuted_csc(const rb_plan *plan, int *colptr, int *rowidx,
                                double *val);
 
+/* ---- Matrix Market input and the benchmark driver's factor (host only;
+ * no device is touched).  These restate the reference program's host side,
+ * so that a maintainer's harness can read the same .mtx files and build the
+ * same triangular factor before calling the solver. */
+
+/* Reader, the reference's mmio_info / mmio_data
+ * (recblock-sptrsv/mmio_highlevel.h:7-137, 139-296): coordinate files of
+ * real / integer / pattern / complex (real part) values; symmetric and
+ * hermitian files are mirrored.  rb_mmio_info gives the CSR sizes (nnz counts
+ * mirrored entries); rb_mmio_data fills caller-allocated CSR arrays (m + 1,
+ * nnz, nnz).  Columns inside a row keep the file's order (mirrored pairs
+ * interleaved as read), exactly as the reference builds them.  Returns 0,
+ * -1 (cannot open / null argument), -2 (banner: not a coordinate matrix of a
+ * known type), -4 (size line), -5 (an entry is missing, malformed or out of
+ * range -- where the reference writes out of bounds). */
+int rb_mmio_info(int *m, int *n, int *nnz, int *is_symmetric, const char *path);
+int rb_mmio_data(int *csr_rowptr, int *csr_colidx, double *csr_val,
+                 const char *path);
+
+/* The triangular factor and right-hand sides of the reference's driver
+ * (recblock-sptrsv/main.cu:150-228): from the CSR of A (m rows), forward
+ * keeps entries with column < row (row order) and appends the diagonal;
+ * backward puts the diagonal first and keeps entries with column > row.
+ * reference_values = 1 reproduces the driver's rewrite: every kept entry is
+ * rand() % 10 + 1 and the diagonal 1.0 (C library rand() after
+ * srand(seed)); otherwise the file's values are kept and the row's diagonal
+ * entries (summed) must exist (RB_ERR_MISSING_DIAG).  Then, for nrhs > 0,
+ * x_ref[r*m + i] = rand() % 10 + 1 and b = T * x_ref per column (T's row
+ * order).  Call with tri_colidx = NULL first to get tri_nnz (tri_rowptr has
+ * m + 1 entries); x_ref and b have m * nrhs entries.  Output is CSR: pass it
+ * through a transposition for the solver's CSC. */
+rb_status rb_mmio_triangle(int m, const int *rowptr, const int *colidx,
+                           const double *val, int substitution,
+                           int reference_values, unsigned seed, int nrhs,
+                           int *tri_rowptr, int *tri_colidx, double *tri_val,
+                           int *tri_nnz, double *x_ref, double *b);
+
 #ifdef __cplusplus
 }
 #endif

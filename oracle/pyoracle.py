@@ -212,3 +212,139 @@ def full_pipeline(colptr, rowidx, val, m, subst, lv):
     bnz = block_nnz(re["colptr"], re["rowidx"], lv, subst, part)
     an = analyze(re["colptr"], re["rowidx"], m, lv, subst, part, bnz)
     return re, part, bnz, an
+
+
+# ---------------------------------------------------------------- Matrix
+# Market input and the driver's factor (pure Python: small files only)
+
+def mm_read_csr(path):
+    """The reference's mmio_info + mmio_data
+    (recblock-sptrsv/mmio_highlevel.h:7-137, 139-296): banner, comment lines,
+    "M N nz", then nz entries "i j [value [imag]]" (1-based); real / integer
+    / pattern (value 1.0) / complex (real part).  Symmetric and hermitian
+    files are mirrored.  CSR: per-row counts (an off-diagonal entry of a
+    symmetric file also counts for its column's row), exclusive scan, then
+    one pass in file order writing at each row's next position, a mirrored
+    pair as (i, j) then (j, i).  Returns (m, n, rowptr, colidx, val,
+    is_symmetric)."""
+    with open(path) as fh:
+        lines = fh.read().split("\n")
+    banner = lines[0].split()
+    kind, sym = banner[3].lower(), banner[4].lower()
+    symmetric = sym in ("symmetric", "hermitian")
+    k = 1
+    while lines[k].strip() == "" or lines[k].lstrip().startswith("%"):
+        k += 1
+    m, n, nz = (int(t) for t in lines[k].split()[:3])
+    k += 1
+    ent = []
+    while len(ent) < nz:
+        t = lines[k].split()
+        k += 1
+        if not t or t[0].startswith("%"):
+            continue
+        i, j = int(t[0]) - 1, int(t[1]) - 1
+        if kind == "pattern":
+            v = 1.0
+        elif kind == "integer":
+            v = float(int(t[2]))
+        else:
+            v = float(t[2])
+        ent.append((i, j, v))
+    cnt = [0] * (m + 1)
+    for i, j, _ in ent:
+        cnt[i + 1] += 1
+        if symmetric and i != j:
+            cnt[j + 1] += 1
+    for r in range(m):
+        cnt[r + 1] += cnt[r]
+    rowptr = np.array(cnt, np.int32)
+    nxt = list(cnt[:m])
+    colidx = np.zeros(cnt[m], np.int32)
+    val = np.zeros(cnt[m], np.float64)
+    for i, j, v in ent:
+        colidx[nxt[i]], val[nxt[i]] = j, v
+        nxt[i] += 1
+        if symmetric and i != j:
+            colidx[nxt[j]], val[nxt[j]] = i, v
+            nxt[j] += 1
+    return m, n, rowptr, colidx, val, symmetric
+
+
+def c_rand(seed, count):
+    """count values of the C library's rand() after srand(seed) (the
+    reference draws its values with it, main.cu:141,166,187,213)."""
+    libc = ctypes.CDLL("libc.so.6")
+    libc.srand.argtypes = [ctypes.c_uint]
+    libc.rand.restype = ctypes.c_int
+    libc.srand(seed)
+    return [libc.rand() for _ in range(count)]
+
+
+def glibc_random_model(seed, count):
+    """glibc's rand() from its published algorithm (additive feedback,
+    r[i] = r[i-3] + r[i-31] mod 2^32 after a Lehmer seeding of 31 words
+    (seed 0 taken as 1), 344 outputs discarded, each output r >> 1): an
+    independent model to check c_rand against."""
+    r = [0] * (344 + count)
+    r[0] = (seed & 0xffffffff) or 1  # a zero seed is replaced by 1
+    if r[0] >= 2 ** 31:  # the seed is used as a signed 32-bit word
+        r[0] -= 2 ** 32
+    for i in range(1, 31):
+        # C division and remainder (truncating: the first word may be
+        # negative)
+        hi = r[i - 1] // 127773 if r[i - 1] >= 0 else -((-r[i - 1]) // 127773)
+        lo = r[i - 1] - hi * 127773
+        w = 16807 * lo - 2836 * hi
+        r[i] = w + 2147483647 if w < 0 else w
+    for i in range(31, 34):
+        r[i] = r[i - 31]
+    for i in range(34, 344 + count):
+        r[i] = (r[i - 31] + r[i - 3]) & 0xffffffff
+    return [(r[i] & 0xffffffff) >> 1 for i in range(344, 344 + count)]
+
+
+def mm_triangle(m, rowptr, colidx, val, subst, reference_values, seed, nrhs):
+    """The driver's factor (recblock-sptrsv/main.cu:150-228): forward keeps
+    columns < row (row order) and appends the diagonal; backward puts the
+    diagonal first, then columns > row.  reference_values: kept entries
+    rand() % 10 + 1 and diagonal 1.0 (rand after srand(seed)); else the
+    file's values with the row's diagonal entries summed (ValueError when a
+    row has none).  Then x_ref = rand() % 10 + 1 (m * nrhs, column-major)
+    and b = T * x_ref per column in T's row order.  Returns (rowptr, colidx,
+    val, x_ref, b) of T in CSR."""
+    rows = []
+    nkept = 0
+    for i in range(m):
+        cols = colidx[rowptr[i]:rowptr[i + 1]]
+        kept = [k for k in range(rowptr[i], rowptr[i + 1])
+                if (colidx[k] < i if subst == 0 else colidx[k] > i)]
+        nkept += len(kept)
+        rows.append(kept)
+        if not reference_values and not (cols == i).any():
+            raise ValueError("row %d has no diagonal" % i)
+    draws = iter(c_rand(seed, nkept + m * nrhs))
+    tp, tc, tv = [0], [], []
+    for i in range(m):
+        d = 1.0 if reference_values else float(
+            sum(val[k] for k in range(rowptr[i], rowptr[i + 1]) if colidx[k] == i))
+        if subst == 1:
+            tc.append(i)
+            tv.append(d)
+        for k in rows[i]:
+            tc.append(int(colidx[k]))
+            tv.append(float(next(draws) % 10 + 1) if reference_values else float(val[k]))
+        if subst == 0:
+            tc.append(i)
+            tv.append(d)
+        tp.append(len(tc))
+    xref = np.array([float(next(draws) % 10 + 1) for _ in range(m * nrhs)])
+    b = np.zeros(m * nrhs)
+    for r in range(nrhs):
+        for i in range(m):
+            s = 0.0
+            for k in range(tp[i], tp[i + 1]):
+                s += tv[k] * xref[r * m + tc[k]]
+            b[r * m + i] = s
+    return (np.array(tp, np.int32), np.array(tc, np.int32), np.array(tv),
+            xref, b)

@@ -107,12 +107,18 @@ constexpr int fifo_groups(int K) {
 }
 static_assert(2 * 2048 * 8 <= (1 << kBlockShift), "group count bits");
 
+// record words per lane for a format
+constexpr int words_of(int ndpp, int nring) {
+    return (2 + (ndpp ? 6 : 0) + ((nring + 1) & ~1) + 2 * nring + 3) / 4;
+}
+
 template <int NDPP, int NRING>
 struct Fmt {
     static constexpr int kSrc = 2 + (NDPP ? 6 : 0);
     static constexpr int kVal = kSrc + ((NRING + 1) & ~1);
     static constexpr int kInts = kVal + 2 * NRING;
-    static constexpr int K = (kInts + 3) / 4;
+    static constexpr int K = words_of(NDPP, NRING);
+    static_assert(K == (kInts + 3) / 4, "record width");
     static constexpr int D = prefetch_steps(K);
     static constexpr int NB = D + 1;  // record buffers, steps per block
     static constexpr int FF = fifo_groups(K);
@@ -138,11 +144,6 @@ __device__ inline double ld_x(const double *p) {
     return __longlong_as_double((long long)__hip_atomic_load(
         (const unsigned long long *)p, __ATOMIC_RELAXED,
         __HIP_MEMORY_SCOPE_AGENT));
-}
-__device__ inline void st_x(double *p, double v) {
-    __hip_atomic_store((unsigned long long *)p,
-                       (unsigned long long)__double_as_longlong(v),
-                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ inline bool pending(double v) {
     return __double_as_longlong(v) == (long long)kPending;
@@ -560,10 +561,18 @@ __device__ __forceinline__ void resolve(const FusedArgs &a, int w, int r,
     }
 }
 
+// Waves per SIMD the register allocation aims at: 3 (6 workgroups per
+// compute unit, for launches of several right-hand sides) for the
+// block-checked kernels, unless their LDS (a 2048-group FIFO plus the ring)
+// allows only 2; the per-step-checked kernels keep their registers.
+constexpr int occupancy_target(int ndpp, int nring, bool blk) {
+    return blk && !(nring > 0 && fifo_groups(words_of(ndpp, nring)) > 1024) ? 3 : 1;
+}
+
 // One workgroup per (consumer wave w, right-hand side r): blockIdx.x =
 // r * W + w.  Right-hand sides share the records and nothing else.
 template <int NDPP, int NRING, bool BLK>
-__global__ void __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(BLK ? 3 : 1))) k_fused(FusedArgs a) {
+__global__ void __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(occupancy_target(NDPP, NRING, BLK)))) k_fused(FusedArgs a) {
     __shared__ double xring[NRING > 0 ? kR * 64 + 1 : 1];
     constexpr int FF = Fmt<NDPP, NRING>::FF;
     __shared__ double fifo[FF + 1];  // + the zero slot
@@ -651,10 +660,7 @@ struct FusedPlan {
 
 static int ring_class(int e) { return e == 0 ? 0 : (e <= 2 ? 2 : (e <= 4 ? 4 : 8)); }
 
-static int words_for(int ndpp, int nring) {
-    const int ints = 2 + (ndpp ? 6 : 0) + ((nring + 1) & ~1) + 2 * nring;
-    return (ints + 3) / 4;
-}
+static int words_for(int ndpp, int nring) { return words_of(ndpp, nring); }
 
 static rb_status ensure_rhs(FusedPlan *F, int nr, hipStream_t s);
 

@@ -23,6 +23,7 @@ EXPORTS = [
     "rb_plan_set_timing", "rb_plan_kernel_ms", "rb_solve_f32ir",
     "rb_batch_create", "rb_batch_solve", "rb_batch_solve_f32ir",
     "rb_batch_plan", "rb_batch_destroy",
+    "rb_mmio_info", "rb_mmio_data", "rb_mmio_triangle",
 ]
 
 RB_OK = 0
@@ -136,6 +137,14 @@ def lib():
     L.recblocking_solver.argtypes = [vp, vp, vp, i, i, i, vp, vp, vp, i, i, i,
                                      ctypes.POINTER(ctypes.c_double)]
     L.recblocking_solver.restype = None
+    cp = ctypes.c_char_p
+    L.rb_mmio_info.argtypes = [ctypes.POINTER(i)] * 4 + [cp]
+    L.rb_mmio_info.restype = i
+    L.rb_mmio_data.argtypes = [vp, vp, vp, cp]
+    L.rb_mmio_data.restype = i
+    L.rb_mmio_triangle.argtypes = [i, vp, vp, vp, i, i, ctypes.c_uint, i,
+                                   vp, vp, vp, ctypes.POINTER(i), vp, vp]
+    L.rb_mmio_triangle.restype = i
     _lib = L
     return L
 
