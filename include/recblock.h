@@ -59,7 +59,14 @@ typedef struct {
                           rb_solve_f32ir */
     int tune_rhs;      /* RB_SCHED_AUTO: right-hand sides per call the
                           schedule is tuned for (0 or 1: single) */
-    int reserved[2];
+    int canonicalize;  /* 1 = accept row ids in any order inside a column
+                          and repeated entries (summed in input order): the
+                          plan is built from the sorted, merged matrix and
+                          its nnz is the merged count.  0 (default) =
+                          RB_ERR_BAD_STRUCTURE for such input, as before.
+                          No counterpart in the reference, which needs
+                          sorted, unique row ids. */
+    int reserved[1];
 } rb_opts;
 
 typedef struct rb_plan rb_plan;

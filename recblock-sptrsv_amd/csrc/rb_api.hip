@@ -168,8 +168,27 @@ rb_status rb_plan_create(const int *d_colptr, const int *d_rowidx,
     p->graph_failed = o.no_graph != 0;  // eager launches only
     p->mixed = o.mixed != 0;
     p->tune_rhs = o.tune_rhs;
-    rb_status st = build_plan(p, d_colptr, d_rowidx, d_val,
-                              (hipStream_t)stream);
+    rb_status st = RB_OK;
+    Arena canon;  // the canonical copy lives until the plan is built
+    struct CanonGuard {
+        Arena &a;
+        ~CanonGuard() { a.release(); }
+    } canon_guard{canon};
+    if (o.canonicalize) {
+        int *cp = nullptr, *ri = nullptr;
+        double *cv = nullptr;
+        int cnnz = 0;
+        st = canonicalize_csc(d_colptr, d_rowidx, d_val, m, nnz, canon, &cp,
+                              &ri, &cv, &cnnz, (hipStream_t)stream);
+        if (st == RB_OK) {
+            d_colptr = cp;
+            d_rowidx = ri;
+            d_val = cv;
+            p->nnz = cnnz;
+        }
+    }
+    if (st == RB_OK)
+        st = build_plan(p, d_colptr, d_rowidx, d_val, (hipStream_t)stream);
     if (st == RB_OK) st = choose_schedule(p, (hipStream_t)stream);
     if (st != RB_OK) {
         (void)hipStreamSynchronize((hipStream_t)stream);

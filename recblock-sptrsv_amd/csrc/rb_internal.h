@@ -164,6 +164,12 @@ struct rb_plan {
 };
 
 namespace rb {
+// Sorted, merged copy of a CSC (rb_canon.hip), allocated in `out`.
+rb_status canonicalize_csc(const int *d_colptr, const int *d_rowidx,
+                           const double *d_val, int n, int nnz, Arena &out,
+                           int **c_colptr, int **c_rowidx, double **c_val,
+                           int *c_nnz, hipStream_t s);
+
 rb_status build_plan(rb_plan *p, const int *d_colptr, const int *d_rowidx,
                      const double *d_val, hipStream_t s);
 rb_status solve_blocked(rb_plan *p, const double *d_b, int ldb, double *d_x,

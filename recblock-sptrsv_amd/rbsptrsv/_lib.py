@@ -45,7 +45,8 @@ class rb_opts(ctypes.Structure):
     _fields_ = [("schedule", ctypes.c_int), ("device_id", ctypes.c_int),
                 ("fused_waves", ctypes.c_int), ("no_graph", ctypes.c_int),
                 ("mixed", ctypes.c_int), ("tune_rhs", ctypes.c_int),
-                ("reserved", ctypes.c_int * 2)]
+                ("canonicalize", ctypes.c_int),
+                ("reserved", ctypes.c_int * 1)]
 
 
 class rb_plan_info(ctypes.Structure):

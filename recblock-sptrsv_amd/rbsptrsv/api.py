@@ -50,7 +50,8 @@ class Plan:
 
     def __init__(self, colptr, rowidx, val, substitution=0, lv=-1,
                  schedule=_lib.SCHED_AUTO, stream=None, device_id=0,
-                 fused_waves=0, graph=True, mixed=False, tune_rhs=1):
+                 fused_waves=0, graph=True, mixed=False, tune_rhs=1,
+                 canonicalize=False):
         torch = _torch()
         self._keep = []
         dcp = colptr if torch.is_tensor(colptr) else _dev_i32(colptr)
@@ -66,6 +67,7 @@ class Plan:
         opts.no_graph = 0 if graph else 1
         opts.mixed = 1 if mixed else 0
         opts.tune_rhs = tune_rhs
+        opts.canonicalize = 1 if canonicalize else 0
         h = ctypes.c_void_p()
         check(lib().rb_plan_create(_ptr(dcp), _ptr(dri), _ptr(dv), self.m,
                                    self.m, self.nnz, substitution, lv,
@@ -74,6 +76,7 @@ class Plan:
         self._h = h
         self.substitution = substitution
         info = self.info()
+        self.nnz = info.nnz  # the merged count when canonicalized
         self.lv = info.lv
         self.nblocks = info.nblocks
         self.nlevel = info.nlevel

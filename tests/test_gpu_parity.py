@@ -248,6 +248,84 @@ def test_error_codes_not_hangs(torch):
     _expect_status(torch, [0, 1, 3], [0, 1, 0], [1.0] * 3, 1, 3)
 
 
+# ---------------------------------------- general inputs (canonicalize)
+def _scramble(f, seed, dup_frac=0.3):
+    """The same matrix with row ids shuffled inside every column and a
+    fraction of the entries split into two or three repeated entries whose
+    values add up to the original (exact halves / thirds are not needed:
+    the split parts are a and v - a)."""
+    rng = np.random.default_rng(seed)
+    cp, ri, v = [0], [], []
+    for j in range(f.n):
+        rows = list(f.rowidx[f.colptr[j]:f.colptr[j + 1]])
+        vals = list(f.val[f.colptr[j]:f.colptr[j + 1]])
+        er, ev = [], []
+        for r, x in zip(rows, vals):
+            k = 1 + (rng.random() < dup_frac) + (rng.random() < dup_frac / 3)
+            parts = [float(rng.uniform(-1, 1)) for _ in range(k - 1)]
+            parts.append(x - sum(parts))
+            er += [r] * k
+            ev += parts
+        perm = rng.permutation(len(er))
+        ri += [er[q] for q in perm]
+        v += [ev[q] for q in perm]
+        cp.append(len(ri))
+    return (np.array(cp, np.int32), np.array(ri, np.int32), np.array(v))
+
+
+@pytest.mark.parametrize("name,make,lower", [
+    ("random3000", lambda: gen.random_lower(3000, 4, seed=21), True),
+    ("laplacian40", lambda: gen.laplacian(40), True),
+    ("banded5000-upper", lambda: gen.banded(5000, seed=2).transpose(), False),
+])
+@pytest.mark.parametrize("schedule", [1, 2])
+def test_canonicalize_general_input(oracle, torch, name, make, lower, schedule):
+    """Unsorted row ids and repeated entries are rejected by default and,
+    with canonicalize, solved as the merged matrix: x within 1e-10 of the
+    oracle's solve of the original factor, plan nnz = the original nnz."""
+    from rbsptrsv import Plan, RbError
+    f = make()
+    subst = 0 if lower else 1
+    cp, ri, v = _scramble(f, 5)
+    print("canonicalize", name, "schedule", schedule, "entries", len(ri),
+          "->", f.nnz, flush=True)
+    with pytest.raises(RbError) as ei:
+        Plan(cp, ri, v, subst, -1, schedule=schedule)
+    assert ei.value.status == 3
+    p = Plan(cp, ri, v, subst, -1, schedule=schedule, canonicalize=True)
+    assert p.nnz == f.nnz
+    b = gen.rhs(f.n, seed=8)
+    x = p.solve(torch.as_tensor(b, device="cuda")).cpu().numpy()
+    p.check()
+    lvv = oracle.lv_auto(f.n)
+    re, part, bnz, an = oracle.full_pipeline(f.colptr, f.rowidx, f.val, f.n,
+                                             subst, lvv)
+    xo = oracle.solve_blocks(re, f.n, lvv, subst, part, b)
+    assert rel_err(x, xo) < X_TOL
+    # the merged matrix is the original one: same permuted CSC as a plan of
+    # the original factor
+    q = Plan.from_factor(f, -1, schedule=schedule)
+    assert np.array_equal(p.permuted_csc()[0], q.permuted_csc()[0])
+    assert np.array_equal(p.permuted_csc()[1], q.permuted_csc()[1])
+    assert np.allclose(p.permuted_csc()[2], q.permuted_csc()[2], rtol=1e-14,
+                       atol=1e-15)
+
+
+def test_canonicalize_still_checks_structure(torch):
+    """Canonicalizing does not hide real errors: a row id out of range, an
+    entry outside the triangle, a missing diagonal."""
+    from rbsptrsv import Plan, RbError
+    cases = [([0, 2, 3], [0, 5, 1], [1.0, 1.0, 1.0], 0, 3),
+             ([0, 1, 3], [0, 1, 0], [1.0, 1.0, 1.0], 0, 3),
+             ([0, 1, 1, 2], [0, 2], [1.0, 1.0], 0, 4),
+             ([0, 2, 3], [0, 0, 1], [1.0, -1.0, 1.0], 0, 5)]  # 1 + -1 = 0
+    for cp, ri, v, subst, code in cases:
+        with pytest.raises(RbError) as ei:
+            Plan(np.asarray(cp), np.asarray(ri), np.asarray(v), subst, 1,
+                 canonicalize=True)
+        assert ei.value.status == code, (cp, ri, v)
+
+
 # ------------------------------------------------ legacy entry points
 def test_legacy_recblocking_solver_cuda(oracle, torch):
     from rbsptrsv import recblocking_solver_cuda, set_bench_repeat
