@@ -829,15 +829,25 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
     auto is_loc = [&](int q, int u) {
         return wave[u] == wave[q] && stepw[q] - stepw[u] <= kR;
     };
-    long n_all = 0, n_dpp = 0;
+    long n_all = 0, n_dpp = 0, n_loc = 0;
     for (int q = 0; q < m; q++) {
         const int r = pof(q);
         for (int e = rp[r]; e < rp[r + 1]; e++) {
             if (cc[e] == r) continue;
             n_all++;
             n_dpp += is_dpp(q, qof(cc[e]));
+            n_loc += is_loc(q, qof(cc[e]));
         }
     }
+    // A schedule whose entries are mostly cross-wave runs at the resolver's
+    // pace and loses to the blocked schedule on one right-hand side (the
+    // tuner's measurements on banded and power-law factors: 3-12x slower).
+    // When the caller leaves the choice to the tuner for single right-hand
+    // sides, do not build it: its build is the most expensive part of the
+    // plan on such factors (1.5 s of 1.7 s on the 4M-row power-law case).
+    if (p->want_schedule == RB_SCHED_AUTO && p->tune_rhs <= 1 &&
+        4 * (n_all - n_loc) > n_all)
+        return RB_OK;
     const int NDPP = (n_dpp > 0 && 4 * n_dpp >= n_all) ? 3 : 0;
     int emax = 0;
     for (int q = 0; q < m; q++) {
