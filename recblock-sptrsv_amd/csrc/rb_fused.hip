@@ -38,10 +38,12 @@
 // stops instead of hanging.
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "rb_internal.h"
@@ -881,22 +883,25 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
     std::vector<int4> rec((size_t)NS * K * 64);
     std::vector<int> origv((size_t)NS * 64, -1);
     std::vector<double> sinvd((size_t)NS * 64, 0.0);
-    std::vector<int> gsrc, gmeta;
-    std::vector<double> gval;
-    std::vector<char> needpub(m, 0);
-    std::vector<int> f(4 * K);
-    auto put = [&](int at, double v) {
-        long long bits;
-        std::memcpy(&bits, &v, 8);
-        f[at] = (int)(bits & 0xffffffff);
-        f[at + 1] = (int)(bits >> 32);
-    };
     const int NB = block_steps(K), FF = fifo_groups(K);
-    // (waiting block, producing block) of every global entry; waves start
-    // at multiples of NB, so block = global step / NB
-    std::vector<std::pair<int, int>> wait_edges;
-    for (int w = 0; w < W; w++) {
-        gent0[w] = (int)gsrc.size();
+    // Per wave, in parallel on host threads (waves write disjoint record,
+    // slot and 1/diag ranges; their global entries, sources and block
+    // wait edges are kept per wave and joined in wave order below).
+    struct WaveOut {
+        std::vector<int> gsrc, gmeta, srcq;
+        std::vector<double> gval;
+        std::vector<std::pair<int, int>> edges;
+    };
+    std::vector<WaveOut> wout(W);
+    auto emit_wave = [&](int w) {
+        WaveOut &o = wout[w];
+        std::vector<int> f(4 * K);
+        auto put = [&](int at, double v) {
+            long long bits;
+            std::memcpy(&bits, &v, 8);
+            f[at] = (int)(bits & 0xffffffff);
+            f[at + 1] = (int)(bits >> 32);
+        };
         int gord = 0, gofs = 0;
         size_t bfirst = 0;
         for (int si = 0; si < (int)steps[w].size(); si++) {
@@ -926,17 +931,17 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
                             put(kVal + 2 * nl, -cv[e] * invd[r]);
                             nl++;
                         } else {
-                            needpub[u] = 1;
-                            wait_edges.emplace_back((step0[w] + si) / NB,
-                                                    slot_of(u) / 64 / NB);
-                            gsrc.push_back(slot_of(u));
-                            gval.push_back(cv[e] * invd[r]);
-                            gmeta.push_back(gord * 2);
+                            o.srcq.push_back(u);
+                            o.edges.emplace_back((step0[w] + si) / NB,
+                                                 slot_of(u) / 64 / NB);
+                            o.gsrc.push_back(slot_of(u));
+                            o.gval.push_back(cv[e] * invd[r]);
+                            o.gmeta.push_back(gord * 2);
                             hasg = true;
                         }
                     }
                     if (hasg) {
-                        gmeta.back() |= 1;
+                        o.gmeta.back() |= 1;
                         gord++;
                     }
                     sinvd[(size_t)(step0[w] + si) * 64 + l] = invd[r];
@@ -952,6 +957,32 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
             // the block's group count goes to lane 0 of its first step
             if (si % NB == NB - 1) rec[bfirst].x |= gofs << kBlockShift;
         }
+    };
+    {
+        const int nth = std::max(1, std::min({W, 16, (int)std::thread::hardware_concurrency()}));
+        std::vector<std::thread> pool;
+        std::atomic<int> next{0};
+        for (int t = 0; t < nth; t++)
+            pool.emplace_back([&]() {
+                for (int w; (w = next.fetch_add(1)) < W;) emit_wave(w);
+            });
+        for (auto &t : pool) t.join();
+    }
+    std::vector<int> gsrc, gmeta;
+    std::vector<double> gval;
+    std::vector<char> needpub(m, 0);
+    // (waiting block, producing block) of every global entry; waves start
+    // at multiples of NB, so block = global step / NB
+    std::vector<std::pair<int, int>> wait_edges;
+    for (int w = 0; w < W; w++) {
+        const WaveOut &o = wout[w];
+        gent0[w] = (int)gsrc.size();
+        gsrc.insert(gsrc.end(), o.gsrc.begin(), o.gsrc.end());
+        gval.insert(gval.end(), o.gval.begin(), o.gval.end());
+        gmeta.insert(gmeta.end(), o.gmeta.begin(), o.gmeta.end());
+        wait_edges.insert(wait_edges.end(), o.edges.begin(), o.edges.end());
+        for (int u : o.srcq) needpub[u] = 1;
+        wout[w] = WaveOut();
     }
     gent0[W] = (int)gsrc.size();
     const bool blk = wait_graph_acyclic(NS / NB, step0, NB, wait_edges);
