@@ -65,6 +65,13 @@ __device__ inline int block_excl_scan(int v, int *lds, int *total) {
 __device__ inline int ld_relaxed(const int *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// workgroup scope: while one workgroup walks narrow levels it is the only
+// reader and writer of rem / par / levelItem (every other workgroup waits
+// at the grid barrier, whose device-scope release / acquire publishes the
+// walk's results and makes earlier writes visible to it)
+__device__ inline int ld_wg(const int *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 
 // Grid-wide exclusive scan of cnt[0..len) in place (contiguous chunk per
 // block).  Returns the grand total to every thread.  Uses two grid syncs.
@@ -114,8 +121,8 @@ constexpr int kNarrow = 4096;  // frontier size handled by one workgroup
 
 // Consecutive narrow levels inside workgroup 0: the same A / B / scan / C
 // phases, separated by workgroup barriers.  Frontier counts live in LDS.
-// rem / par / levelItem are read with agent-scope loads (they are updated
-// by device-scope atomics and plain stores from other waves).
+// rem / par / levelItem are read, updated and written at workgroup scope
+// (see ld_wg).
 __device__ void bfs_narrow(const int *__restrict__ colptr,
                            const int *__restrict__ rowidx, int m, int *rem,
                            int *par, int *levelItem, int *levelPtr,
@@ -128,12 +135,14 @@ __device__ void bfs_narrow(const int *__restrict__ colptr,
     while (e < m && e - s <= kNarrow) {
         const int len = e - s;
         for (int p = s + tid; p < e; p += blockDim.x) {
-            const int u = ld_relaxed(&levelItem[p]);
+            const int u = ld_wg(&levelItem[p]);
             for (int q = colptr[u]; q < colptr[u + 1]; q++) {
                 const int v = rowidx[q];
                 if (v == u) continue;
-                atomicSub(&rem[v], 1);
-                atomicMax(&par[v], p);
+                __hip_atomic_fetch_add(&rem[v], -1, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_fetch_max(&par[v], p, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -141,12 +150,12 @@ __device__ void bfs_narrow(const int *__restrict__ colptr,
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         for (int i = tid; i < len; i += blockDim.x) {
             const int p = s + i;
-            const int u = ld_relaxed(&levelItem[p]);
+            const int u = ld_wg(&levelItem[p]);
             int c = 0;
             for (int q = colptr[u]; q < colptr[u + 1]; q++) {
                 const int v = rowidx[q];
-                if (v != u && ld_relaxed(&rem[v]) == 0 &&
-                    ld_relaxed(&par[v]) == p)
+                if (v != u && ld_wg(&rem[v]) == 0 &&
+                    ld_wg(&par[v]) == p)
                     c++;
             }
             cl[i] = c;
@@ -166,14 +175,14 @@ __device__ void bfs_narrow(const int *__restrict__ colptr,
         __syncthreads();
         for (int i = tid; i < len; i += blockDim.x) {
             const int p = s + i;
-            const int u = ld_relaxed(&levelItem[p]);
+            const int u = ld_wg(&levelItem[p]);
             int o = e + cl[i];
             for (int q = colptr[u]; q < colptr[u + 1]; q++) {
                 const int v = rowidx[q];
-                if (v != u && ld_relaxed(&rem[v]) == 0 &&
-                    ld_relaxed(&par[v]) == p)
+                if (v != u && ld_wg(&rem[v]) == 0 &&
+                    ld_wg(&par[v]) == p)
                     __hip_atomic_store(&levelItem[o++], v, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         }
         if (tid == 0) levelPtr[nl + 1] = e + total;
