@@ -563,18 +563,23 @@ __device__ __forceinline__ void resolve(const FusedArgs &a, int w, int r,
     }
 }
 
-// Waves per SIMD the register allocation aims at: 3 (6 workgroups per
-// compute unit, for launches of several right-hand sides) for the
-// block-checked kernels, unless their LDS (a 2048-group FIFO plus the ring)
-// allows only 2; the per-step-checked kernels keep their registers.
-constexpr int occupancy_target(int ndpp, int nring, bool blk) {
-    return blk && !(nring > 0 && fifo_groups(words_of(ndpp, nring)) > 1024) ? 3 : 1;
+// Waves per SIMD the register allocation aims at.  MULTI (launches of
+// several right-hand sides, where co-resident workgroups decide how many
+// columns one launch takes): 3, i.e. 6 workgroups per compute unit, for the
+// block-checked kernels unless their LDS (a 2048-group FIFO plus the ring)
+// allows only 2.  Single right-hand sides and per-step-checked kernels keep
+// their registers (a tighter allocation costs the one wave ~2 % per step).
+constexpr int occupancy_target(int ndpp, int nring, bool blk, bool multi) {
+    return multi && blk && !(nring > 0 && fifo_groups(words_of(ndpp, nring)) > 1024)
+               ? 3 : 1;
 }
 
 // One workgroup per (consumer wave w, right-hand side r): blockIdx.x =
 // r * W + w.  Right-hand sides share the records and nothing else.
-template <int NDPP, int NRING, bool BLK>
-__global__ void __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(occupancy_target(NDPP, NRING, BLK)))) k_fused(FusedArgs a) {
+template <int NDPP, int NRING, bool BLK, bool MULTI>
+__global__ void __launch_bounds__(kWG)
+    __attribute__((amdgpu_waves_per_eu(occupancy_target(NDPP, NRING, BLK, MULTI))))
+    k_fused(FusedArgs a) {
     __shared__ double xring[NRING > 0 ? kR * 64 + 1 : 1];
     constexpr int FF = Fmt<NDPP, NRING>::FF;
     __shared__ double fifo[FF + 1];  // + the zero slot
@@ -1096,19 +1101,21 @@ int fused_max_waves() {
 }
 
 template <int NDPP, int NRING>
-static const void *kernel_of(bool blk) {
-    return blk ? (const void *)k_fused<NDPP, NRING, true>
-               : (const void *)k_fused<NDPP, NRING, false>;
+static const void *kernel_of(bool blk, bool multi) {
+    if (!blk) return (const void *)k_fused<NDPP, NRING, false, false>;
+    return multi ? (const void *)k_fused<NDPP, NRING, true, true>
+                 : (const void *)k_fused<NDPP, NRING, true, false>;
 }
 
-static const void *pick_kernel(const FusedPlan *F) {
+// the kernel for F's record format; multi: a launch of several columns
+static const void *pick_kernel(const FusedPlan *F, bool multi) {
     const int r = F->NRING;
     const bool b = F->blk;
     if (F->NDPP)
-        return r == 0 ? kernel_of<3, 0>(b) : r == 2 ? kernel_of<3, 2>(b)
-             : r == 4 ? kernel_of<3, 4>(b) : kernel_of<3, 8>(b);
-    return r == 0 ? kernel_of<0, 0>(b) : r == 2 ? kernel_of<0, 2>(b)
-         : r == 4 ? kernel_of<0, 4>(b) : kernel_of<0, 8>(b);
+        return r == 0 ? kernel_of<3, 0>(b, multi) : r == 2 ? kernel_of<3, 2>(b, multi)
+             : r == 4 ? kernel_of<3, 4>(b, multi) : kernel_of<3, 8>(b, multi);
+    return r == 0 ? kernel_of<0, 0>(b, multi) : r == 2 ? kernel_of<0, 2>(b, multi)
+         : r == 4 ? kernel_of<0, 4>(b, multi) : kernel_of<0, 8>(b, multi);
 }
 
 // Slot-order x buffers for nr right-hand sides (grown on demand; a fresh
@@ -1143,7 +1150,7 @@ static int rhs_per_launch(FusedPlan *F) {
     int dev = 0, ncu = 0, per = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, pick_kernel(F),
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, pick_kernel(F, true),
                                                      kWG, 0) != hipSuccess)
         per = 1;
     (void)hipGetLastError();
@@ -1186,7 +1193,7 @@ rb_status solve_fused(rb_plan *p, const double *d_b, int ldb, double *d_x,
         void *kargs[] = {(void *)&args};
         mark_start(p, s);
         hipError_t e = hipLaunchCooperativeKernel(
-            pick_kernel(F),
+            pick_kernel(F, nr > 1),
             dim3(F->W * nr), dim3(kWG), kargs, 0, s);
         if (e != hipSuccess)
             return status_of_hip(e, "hipLaunchCooperativeKernel(k_fused)");
