@@ -77,13 +77,15 @@ constexpr int no_group_off(int FF) { return FF * 8; }
 // Per-lane record of one step, as 32-bit words; every double starts at an
 // even word so it is an aligned register pair of the load that brings it:
 //   [0] sel (+ group counts)  [1] FIFO offset
-//   NDPP: [2..3] coefficient of lane l (previous step), [4..5] of lane l-1,
-//         [6..7] of lane l+1
+//   NDPP 3: [2..3] coefficient of lane l (previous step), [4..5] of lane
+//           l-1, [6..7] of lane l+1; NDPP 1 (2): [2..3] lane l, [4..5]
+//           lane l-1 (l+1)
 //   NRING ring sources (byte offsets into the ring), padded to even, then
 //   NRING coefficients
 // padded to whole 16-byte words; word k of step s for lane l lives at
-// rec[(s * K + k) * 64 + l].  Every word is read: a loaded word the step
-// does not use would free its registers while the load is in flight.
+// rec[(s * K + k) * 64 + l].  Every loaded word is read (a last word with
+// two ints is loaded as 8 bytes): a loaded word the step does not use would
+// free its registers while the load is in flight.
 // Every coefficient is stored as -a_ij / a_ii, and the global entries'
 // values as a_ij / a_ii, so a row is x_i = b_i/a_ii + sum(coef * x_j) - fifo:
 // fused multiply-adds only, without a subtraction or a product by 1/diag
@@ -109,17 +111,26 @@ constexpr int fifo_groups(int K) {
 }
 static_assert(2 * 2048 * 8 <= (1 << kBlockShift), "group count bits");
 
+// DPP formats: 0 none; 3 lanes l, l-1, l+1 of the previous step; 1 lanes
+// l, l-1 only; 2 lanes l, l+1 only (a grid's forward / backward solve
+// needs one side: one DPP move pair and one FMA less on the chain)
+constexpr int dpp_ints(int ndpp) { return ndpp == 3 ? 6 : ndpp ? 4 : 0; }
+
 // record words per lane for a format
 constexpr int words_of(int ndpp, int nring) {
-    return (2 + (ndpp ? 6 : 0) + ((nring + 1) & ~1) + 2 * nring + 3) / 4;
+    return (2 + dpp_ints(ndpp) + ((nring + 1) & ~1) + 2 * nring + 3) / 4;
 }
 
 template <int NDPP, int NRING>
 struct Fmt {
-    static constexpr int kSrc = 2 + (NDPP ? 6 : 0);
+    static constexpr int kSrc = 2 + dpp_ints(NDPP);
     static constexpr int kVal = kSrc + ((NRING + 1) & ~1);
     static constexpr int kInts = kVal + 2 * NRING;
     static constexpr int K = words_of(NDPP, NRING);
+    // the last word holds two ints only: it is loaded as 8 bytes, so no
+    // loaded register is left unused (the allocator would reuse it while
+    // the load is in flight and have to wait for the load)
+    static constexpr bool kTail2 = kInts % 4 == 2;
     static_assert(K == (kInts + 3) / 4, "record width");
     static constexpr int D = prefetch_steps(K);
     static constexpr int NB = D + 1;  // record buffers, steps per block
@@ -274,10 +285,18 @@ struct Consumer {
     __device__ __forceinline__ void load_rec(int B, int t) {
         const int tc = min(t, last);
 #pragma unroll
-        for (int k = 0; k < K; k++)
-            rb[B][k] = __builtin_bit_cast(
-                int4, __builtin_amdgcn_raw_buffer_load_b128(rec, lane16 + k * 1024,
-                                                             tc * K * 1024, 0));
+        for (int k = 0; k < K; k++) {
+            if (F::kTail2 && k == K - 1) {
+                const int2 v = __builtin_bit_cast(
+                    int2, __builtin_amdgcn_raw_buffer_load_b64(rec, lane16 + k * 1024,
+                                                               tc * K * 1024, 0));
+                rb[B][k] = make_int4(v.x, v.y, 0, 0);
+            } else {
+                rb[B][k] = __builtin_bit_cast(
+                    int4, __builtin_amdgcn_raw_buffer_load_b128(rec, lane16 + k * 1024,
+                                                                 tc * K * 1024, 0));
+            }
+        }
         bb[B] = __builtin_bit_cast(
                 double, __builtin_amdgcn_raw_buffer_load_b64(br, lane8, tc * 512, 0));
     }
@@ -371,15 +390,19 @@ struct Consumer {
                 bv = __builtin_fma(dbl(f[F::kVal + 2 * j], f[F::kVal + 2 * j + 1]),
                                    xv, bv);
             }
-            // the chain that carries the previous step's x: 3 FMAs (the
-            // first needs no DPP)
+            // the chain that carries the previous step's x: one FMA per
+            // neighbour (the first needs no DPP)
             double acc = bv;
-            if constexpr (NDPP > 0) {
+            if constexpr (NDPP == 3) {
                 const double xl = dpp_from_lower(xprev);
                 const double xr = dpp_from_upper(xprev);
                 acc = __builtin_fma(dbl(f[2], f[3]), xprev, acc);
                 acc = __builtin_fma(dbl(f[4], f[5]), xl, acc);
                 acc = __builtin_fma(dbl(f[6], f[7]), xr, acc);
+            } else if constexpr (NDPP == 1 || NDPP == 2) {
+                const double xn = NDPP == 1 ? dpp_from_lower(xprev) : dpp_from_upper(xprev);
+                acc = __builtin_fma(dbl(f[2], f[3]), xprev, acc);
+                acc = __builtin_fma(dbl(f[4], f[5]), xn, acc);
             }
             const double xv = BLK ? acc : acc - fv;
             xprev = xv;
@@ -855,7 +878,21 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
     if (p->want_schedule == RB_SCHED_AUTO && p->tune_rhs <= 1 &&
         4 * (n_all - n_loc) > n_all)
         return RB_OK;
-    const int NDPP = (n_dpp > 0 && 4 * n_dpp >= n_all) ? 3 : 0;
+    // DPP entries by side: lane l-1 (lo) or l+1 (up) of the previous step
+    long n_lo = 0, n_up = 0;
+    for (int q = 0; q < m; q++) {
+        const int r = pof(q);
+        for (int e = rp[r]; e < rp[r + 1]; e++) {
+            if (cc[e] == r) continue;
+            const int u = qof(cc[e]);
+            if (is_dpp(q, u)) {
+                n_lo += lanew[u] < lanew[q];
+                n_up += lanew[u] > lanew[q];
+            }
+        }
+    }
+    const int NDPP = !(n_dpp > 0 && 4 * n_dpp >= n_all) ? 0
+                   : n_up == 0 ? 1 : n_lo == 0 ? 2 : 3;
     int emax = 0;
     for (int q = 0; q < m; q++) {
         const int r = pof(q);
@@ -877,7 +914,7 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
         for (int w = 0; w < W; w++)
             while (steps[w].size() % Bp) steps[w].push_back(empty);
     }
-    const int kSrc = 2 + (NDPP ? 6 : 0), kVal = kSrc + ((NRING + 1) & ~1);
+    const int kSrc = 2 + dpp_ints(NDPP), kVal = kSrc + ((NRING + 1) & ~1);
     // ---- emit records
     std::vector<int> step0(W + 1, 0), gent0(W + 1, 0), pub0(W + 1, 0);
     for (int w = 0; w < W; w++) step0[w + 1] = step0[w] + (int)steps[w].size();
@@ -930,7 +967,10 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
                         const int u = qof(cc[e]);
                         if (NDPP && is_dpp(q, u)) {
                             const int du = lanew[u] - l;  // -1, 0, +1
-                            put(du == 0 ? 2 : (du < 0 ? 4 : 6), -cv[e] * invd[r]);
+                            // lane l at [2..3]; the one side of NDPP 1 / 2
+                            // at [4..5]; NDPP 3: l-1 at [4..5], l+1 at [6..7]
+                            put(du == 0 ? 2 : (du < 0 || NDPP == 2) ? 4 : 6,
+                                -cv[e] * invd[r]);
                         } else if (is_loc(q, u) && nl < NRING) {
                             f[kSrc + nl] = ((stepw[u] % kR) * 64 + lanew[u]) * 8;
                             put(kVal + 2 * nl, -cv[e] * invd[r]);
@@ -1111,9 +1151,15 @@ static const void *kernel_of(bool blk, bool multi) {
 static const void *pick_kernel(const FusedPlan *F, bool multi) {
     const int r = F->NRING;
     const bool b = F->blk;
-    if (F->NDPP)
+    if (F->NDPP == 3)
         return r == 0 ? kernel_of<3, 0>(b, multi) : r == 2 ? kernel_of<3, 2>(b, multi)
              : r == 4 ? kernel_of<3, 4>(b, multi) : kernel_of<3, 8>(b, multi);
+    if (F->NDPP == 1)
+        return r == 0 ? kernel_of<1, 0>(b, multi) : r == 2 ? kernel_of<1, 2>(b, multi)
+             : r == 4 ? kernel_of<1, 4>(b, multi) : kernel_of<1, 8>(b, multi);
+    if (F->NDPP == 2)
+        return r == 0 ? kernel_of<2, 0>(b, multi) : r == 2 ? kernel_of<2, 2>(b, multi)
+             : r == 4 ? kernel_of<2, 4>(b, multi) : kernel_of<2, 8>(b, multi);
     return r == 0 ? kernel_of<0, 0>(b, multi) : r == 2 ? kernel_of<0, 2>(b, multi)
          : r == 4 ? kernel_of<0, 4>(b, multi) : kernel_of<0, 8>(b, multi);
 }
