@@ -42,6 +42,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <numeric>
 #include <thread>
 #include <vector>
@@ -630,27 +631,67 @@ __global__ void __launch_bounds__(kWG)
     }
 }
 
-// b/diag of nr right-hand sides in slot order (0 for padding lanes)
-__global__ void k_fused_prologue(const int *__restrict__ orig,
-                                 const double *__restrict__ sinvd, long nslots,
-                                 int nr, const double *__restrict__ b, long ldb,
-                                 double *brec) {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nslots * nr) return;
-    const long r = i / nslots, q = i - r * nslots;
-    const int o = orig[q];
-    brec[i] = o >= 0 ? b[r * ldb + o] * sinvd[q] : 0.0;
+// b/diag into slot order and x out of it, one tile of kTile consecutive
+// slots per workgroup and one right-hand side per blockIdx.y.  Inside a tile
+// the slots are listed by ascending row (trow; padding lanes last, as -1),
+// tperm holds each listed row's slot offset in the tile and tinvd its
+// 1/diag.  The caller's b and x are then read and written in row order --
+// a chain's rows over consecutive steps are runs of consecutive rows on
+// stencil matrices -- and the slot-order side is a contiguous copy through
+// LDS.
+constexpr int kTile = 1024, kTileThreads = 256;
+// tile offsets sort in the low 12 bits of a key and fit an unsigned short
+static_assert(kTile <= 4096 && (kTile & (kTile - 1)) == 0, "kTile");
+
+__global__ void __launch_bounds__(kTileThreads)
+    k_fused_prologue(const int *__restrict__ trow,
+                     const unsigned short *__restrict__ tperm,
+                     const double *__restrict__ tinvd, long nslots,
+                     const double *__restrict__ b, long ldb, double *brec) {
+    __shared__ double buf[kTile];
+    const long base = (long)blockIdx.x * kTile;
+    const int cnt = (int)min((long)kTile, nslots - base);
+    const double *br = b + blockIdx.y * ldb;
+#pragma unroll
+    for (int k = 0; k < kTile / kTileThreads; k++) {
+        const int i = k * kTileThreads + threadIdx.x;
+        if (i < cnt) {
+            const int row = trow[base + i];
+            buf[tperm[base + i]] = row >= 0 ? br[row] * tinvd[base + i] : 0.0;
+        }
+    }
+    __syncthreads();
+    double *out = brec + blockIdx.y * nslots + base;
+#pragma unroll
+    for (int k = 0; k < kTile / kTileThreads; k++) {
+        const int i = k * kTileThreads + threadIdx.x;
+        if (i < cnt) out[i] = buf[i];
+    }
 }
 
-// slot-order x of nr right-hand sides to the caller's x
-__global__ void k_fused_epilogue(const int *__restrict__ orig, long nslots,
-                                 int nr, const double *__restrict__ xq,
-                                 double *x, long ldx) {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nslots * nr) return;
-    const long r = i / nslots, q = i - r * nslots;
-    const int o = orig[q];
-    if (o >= 0) x[r * ldx + o] = xq[i];
+__global__ void __launch_bounds__(kTileThreads)
+    k_fused_epilogue(const int *__restrict__ trow,
+                     const unsigned short *__restrict__ tperm, long nslots,
+                     const double *__restrict__ xq, double *x, long ldx) {
+    __shared__ double buf[kTile];
+    const long base = (long)blockIdx.x * kTile;
+    const int cnt = (int)min((long)kTile, nslots - base);
+    const double *in = xq + blockIdx.y * nslots + base;
+#pragma unroll
+    for (int k = 0; k < kTile / kTileThreads; k++) {
+        const int i = k * kTileThreads + threadIdx.x;
+        if (i < cnt) buf[i] = in[i];
+    }
+    __syncthreads();
+    double *xr = x + blockIdx.y * ldx;
+#pragma unroll
+    for (int k = 0; k < kTile / kTileThreads; k++) {
+        const int i = k * kTileThreads + threadIdx.x;
+        if (i < cnt) {
+            const int row = trow[base + i];
+            if (row >= 0) xr[row] = buf[tperm[base + i]];
+        }
+    }
 }
 
 __global__ void k_fill_pending(double *p, long n) {
@@ -668,7 +709,10 @@ struct FusedPlan {
     unsigned epoch = 0;  // solves run; selects the slot-order x buffer
     int *d_step0 = nullptr, *d_gent0 = nullptr, *d_pub0 = nullptr;
     int4 *d_rec = nullptr;
-    int *d_orig = nullptr;
+    // slot order <-> row order, per tile of kTile slots (see the prologue)
+    int *d_trow = nullptr;
+    unsigned short *d_tperm = nullptr;
+    double *d_tinvd = nullptr;
     int *d_gsrc = nullptr, *d_gmeta = nullptr, *d_pub = nullptr;
     double *d_gval = nullptr;
     // slot-order x: two buffers of nr_alloc right-hand sides each, and
@@ -678,14 +722,11 @@ struct FusedPlan {
     int nr_alloc = 0;
     std::vector<char> armed[2];
     double *d_brec = nullptr;  // b/diag in slot order, nr_alloc right-hand sides
-    double *d_sinvd = nullptr; // 1/diag in slot order (0 for padding lanes)
-    int nr_cap = 0;            // right-hand sides per launch (co-residency)
+    int ncu = 0, per_cu = 0;   // CUs; k_fused workgroups resident per CU
+    bool small = false;        // single-column solves are ordinary launches
     unsigned long long *d_trace = nullptr;  // RB_FUSED_TRACE=1
     double build_ms = 0.0;
-    ~FusedPlan() {
-        if (d_xq) (void)hipFree(d_xq);
-        if (d_brec) (void)hipFree(d_brec);
-    }
+    ~FusedPlan();
 };
 
 static int ring_class(int e) { return e == 0 ? 0 : (e <= 2 ? 2 : (e <= 4 ? 4 : 8)); }
@@ -693,6 +734,7 @@ static int ring_class(int e) { return e == 0 ? 0 : (e <= 2 ? 2 : (e <= 4 ? 4 : 8
 static int words_for(int ndpp, int nring) { return words_of(ndpp, nring); }
 
 static rb_status ensure_rhs(FusedPlan *F, int nr, hipStream_t s);
+static void device_shape(FusedPlan *F);
 
 // Can every block wait for all of its groups before its first step?  Block
 // X depends on the blocks holding the rows its global entries read and on
@@ -1013,6 +1055,39 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
             });
         for (auto &t : pool) t.join();
     }
+    // every tile's slots listed by ascending row, padding lanes last
+    const size_t nslot_all = (size_t)NS * 64;
+    const int ntile = (int)((nslot_all + kTile - 1) / kTile);
+    std::vector<int> trow(nslot_all);
+    std::vector<unsigned short> tperm(nslot_all);
+    std::vector<double> tinvd(nslot_all);
+    {
+        auto sort_tile = [&](int t) {
+            const size_t base = (size_t)t * kTile;
+            const int cnt = (int)std::min<size_t>(kTile, nslot_all - base);
+            std::vector<unsigned long long> key(cnt);
+            for (int i = 0; i < cnt; i++) {
+                const int o = origv[base + i];
+                const unsigned long long r = o >= 0 ? (unsigned)o : 0xffffffffu;
+                key[i] = r << 12 | (unsigned)i;
+            }
+            std::sort(key.begin(), key.end());
+            for (int i = 0; i < cnt; i++) {
+                const int q = (int)(key[i] & (kTile - 1));
+                tperm[base + i] = (unsigned short)q;
+                trow[base + i] = origv[base + q];
+                tinvd[base + i] = sinvd[base + q];
+            }
+        };
+        const int nth = std::max(1, std::min({ntile, 16, (int)std::thread::hardware_concurrency()}));
+        std::vector<std::thread> pool;
+        std::atomic<int> next{0};
+        for (int t = 0; t < nth; t++)
+            pool.emplace_back([&]() {
+                for (int k; (k = next.fetch_add(1)) < ntile;) sort_tile(k);
+            });
+        for (auto &t : pool) t.join();
+    }
     std::vector<int> gsrc, gmeta;
     std::vector<double> gval;
     std::vector<char> needpub(m, 0);
@@ -1051,7 +1126,8 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
     }
     if (pub.empty()) pub.push_back(0);
     // ---- upload
-    FusedPlan *F = new FusedPlan();
+    std::unique_ptr<FusedPlan> owner(new FusedPlan());
+    FusedPlan *F = owner.get();
     F->W = W;
     F->NDPP = NDPP;
     F->NRING = NRING;
@@ -1066,8 +1142,9 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
     RB_TRY(A.alloc(&F->d_gent0, W + 1));
     RB_TRY(A.alloc(&F->d_pub0, W + 1));
     RB_TRY(A.alloc(&F->d_rec, rec.size()));
-    RB_TRY(A.alloc(&F->d_orig, nslots));
-    RB_TRY(A.alloc(&F->d_sinvd, nslots));
+    RB_TRY(A.alloc(&F->d_trow, nslots));
+    RB_TRY(A.alloc(&F->d_tperm, nslots));
+    RB_TRY(A.alloc(&F->d_tinvd, nslots));
     RB_TRY(A.alloc(&F->d_gsrc, gsrc.size()));
     RB_TRY(A.alloc(&F->d_gval, gval.size()));
     RB_TRY(A.alloc(&F->d_gmeta, gmeta.size()));
@@ -1088,8 +1165,9 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
     RB_TRY(up(F->d_gent0, gent0.data(), sizeof(int) * gent0.size()));
     RB_TRY(up(F->d_pub0, pub0.data(), sizeof(int) * pub0.size()));
     RB_TRY(up(F->d_rec, rec.data(), sizeof(int4) * rec.size()));
-    RB_TRY(up(F->d_orig, origv.data(), sizeof(int) * origv.size()));
-    RB_TRY(up(F->d_sinvd, sinvd.data(), sizeof(double) * sinvd.size()));
+    RB_TRY(up(F->d_trow, trow.data(), sizeof(int) * trow.size()));
+    RB_TRY(up(F->d_tperm, tperm.data(), sizeof(unsigned short) * tperm.size()));
+    RB_TRY(up(F->d_tinvd, tinvd.data(), sizeof(double) * tinvd.size()));
     RB_TRY(up(F->d_gsrc, gsrc.data(), sizeof(int) * gsrc.size()));
     RB_TRY(up(F->d_gval, gval.data(), sizeof(double) * gval.size()));
     RB_TRY(up(F->d_gmeta, gmeta.data(), sizeof(int) * gmeta.size()));
@@ -1097,8 +1175,9 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
     RB_HIP(hipStreamSynchronize(s));
     auto t1 = std::chrono::steady_clock::now();
     F->build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    device_shape(F);
     delete p->fused;
-    p->fused = F;
+    p->fused = owner.release();
     return ensure_rhs(F, 1, s);
 }
 
@@ -1190,9 +1269,24 @@ static rb_status ensure_rhs(FusedPlan *F, int nr, hipStream_t s) {
     return RB_OK;
 }
 
-// Right-hand sides that fit one cooperative launch (W workgroups each).
-static int rhs_per_launch(FusedPlan *F) {
-    if (F->nr_cap > 0) return F->nr_cap;
+// Launch form.  The consumers wait on each other, so all W * nr workgroups
+// of a launch must be resident together.  A cooperative launch guarantees
+// that, but costs about 18 us more per launch than an ordinary one on MI355X
+// (single-column Laplacian 1500^2 solve: 0.291 ms cooperative, 0.273 ms
+// ordinary).  Launches of at most ncu / 8 workgroups -- single-column solves
+// of plans with few waves -- are ordinary launches: such a grid is resident
+// once dispatched as long as the device's other resident work leaves it that
+// many workgroup slots.  The cooperative (multi-column) launches size
+// themselves to leave those slots to up to kSmallStreams other live plans
+// whose solves may be ordinary launches on other streams.  Every wait in the
+// kernel is bounded all the same.
+constexpr int kSmallStreams = 3;
+static std::atomic<int> g_small_plans{0};
+
+static int small_grid(int ncu) { return std::max(1, ncu / 8); }
+
+static void device_shape(FusedPlan *F) {
+    if (F->ncu > 0) return;
     int dev = 0, ncu = 0, per = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
@@ -1200,8 +1294,25 @@ static int rhs_per_launch(FusedPlan *F) {
                                                      kWG, 0) != hipSuccess)
         per = 1;
     (void)hipGetLastError();
-    F->nr_cap = std::max(1, std::min(64, (per * ncu) / std::max(1, F->W)));
-    return F->nr_cap;
+    F->ncu = std::max(1, ncu);
+    F->per_cu = std::max(1, per);
+    F->small = F->W <= small_grid(F->ncu);
+    if (F->small) g_small_plans.fetch_add(1);
+}
+
+FusedPlan::~FusedPlan() {
+    if (small) g_small_plans.fetch_sub(1);
+    if (d_xq) (void)hipFree(d_xq);
+    if (d_brec) (void)hipFree(d_brec);
+}
+
+// Right-hand sides that fit one cooperative launch (W workgroups each).
+static int rhs_per_launch(FusedPlan *F) {
+    device_shape(F);
+    const int others = std::max(
+        0, std::min(kSmallStreams, g_small_plans.load() - (F->small ? 1 : 0)));
+    const int slots = F->per_cu * F->ncu - others * small_grid(F->ncu);
+    return std::max(1, std::min(64, slots / std::max(1, F->W)));
 }
 
 rb_status solve_fused(rb_plan *p, const double *d_b, int ldb, double *d_x,
@@ -1213,6 +1324,7 @@ rb_status solve_fused(rb_plan *p, const double *d_b, int ldb, double *d_x,
     }
     if (p->m == 0) return RB_OK;
     const size_t nslots = (size_t)F->nsteps * 64;
+    device_shape(F);
     const int cap = nrhs > 1 ? rhs_per_launch(F) : 1;
     RB_TRY(ensure_rhs(F, std::min(nrhs, cap), s));
     for (int k0 = 0; k0 < nrhs; k0 += cap) {
@@ -1228,24 +1340,28 @@ rb_status solve_fused(rb_plan *p, const double *d_b, int ldb, double *d_x,
                 RB_HIP(hipGetLastError());
             }
         const double *b = d_b + (size_t)k0 * ldb;
-        const long tot = (long)nslots * nr;
-        const int grid = (int)((tot + 255) / 256);
-        k_fused_prologue<<<grid, 256, 0, s>>>(F->d_orig, F->d_sinvd, (long)nslots,
-                                              nr, b, ldb, F->d_brec);
+        const dim3 tiles((unsigned)((nslots + kTile - 1) / kTile), (unsigned)nr);
+        k_fused_prologue<<<tiles, kTileThreads, 0, s>>>(
+            F->d_trow, F->d_tperm, F->d_tinvd, (long)nslots, b, ldb, F->d_brec);
         RB_HIP(hipGetLastError());
         FusedArgs args{F->d_step0, F->d_gent0, F->d_pub0, F->d_rec,
                        F->d_gsrc, F->d_gval, F->d_gmeta, F->d_pub, cur, nxt,
                        F->d_brec, (long)nslots, F->W, p->d_status, F->d_trace};
         void *kargs[] = {(void *)&args};
         mark_start(p, s);
-        hipError_t e = hipLaunchCooperativeKernel(
-            pick_kernel(F, nr > 1),
-            dim3(F->W * nr), dim3(kWG), kargs, 0, s);
+        const bool ordinary = F->W * nr <= small_grid(F->ncu);
+        const void *kern = pick_kernel(F, nr > 1);
+        hipError_t e =
+            ordinary
+                ? hipLaunchKernel(kern, dim3(F->W * nr), dim3(kWG), kargs, 0, s)
+                : hipLaunchCooperativeKernel(kern, dim3(F->W * nr), dim3(kWG),
+                                             kargs, 0, s);
         if (e != hipSuccess)
-            return status_of_hip(e, "hipLaunchCooperativeKernel(k_fused)");
+            return status_of_hip(e, ordinary ? "hipLaunchKernel(k_fused)"
+                                             : "hipLaunchCooperativeKernel(k_fused)");
         mark_end(p, s);
-        k_fused_epilogue<<<grid, 256, 0, s>>>(F->d_orig, (long)nslots, nr, cur,
-                                              d_x + (size_t)k0 * ldx, ldx);
+        k_fused_epilogue<<<tiles, kTileThreads, 0, s>>>(
+            F->d_trow, F->d_tperm, (long)nslots, cur, d_x + (size_t)k0 * ldx, ldx);
         RB_HIP(hipGetLastError());
         for (int r = 0; r < nr; r++) {
             F->armed[cb][r] = 0;
