@@ -118,6 +118,173 @@ __device__ int grid_excl_scan(cg::grid_group &g, int *cnt, int len,
 }
 
 constexpr int kNarrow = 4096;  // frontier size handled by one workgroup
+constexpr int kIlp = 6;        // frontier nodes per thread, cached levels
+constexpr int kE = 4;          // column entries per node kept in registers
+
+// Exclusive scan of cl[0..len) in place (one workgroup); returns the sum.
+__device__ int narrow_scan(int *cl, int len, int *lds) {
+    const int tid = threadIdx.x;
+    const int per = (len + blockDim.x - 1) / blockDim.x;
+    const int lo = min(len, tid * per), hi = min(len, lo + per);
+    int local = 0;
+    for (int i = lo; i < hi; i++) local += cl[i];
+    int total;
+    int run = block_excl_scan(local, lds, &total);
+    for (int i = lo; i < hi; i++) {
+        const int t = cl[i];
+        cl[i] = run;
+        run += t;
+    }
+    __syncthreads();
+    return total;
+}
+
+// One narrow level [s, e) of at most kIlp * kBfsThreads nodes.  Each thread
+// keeps its nodes, their entry ranges and the first kE entries of their
+// columns in registers from phase A to C, and phase B remembers which of
+// those entries are children, so B reads only rem / par and C only writes:
+// a level costs about four dependent memory round trips instead of a dozen.
+// Entries past the first kE of a column are re-read (the generic rule).
+__device__ int level_cached(const int *__restrict__ colptr,
+                            const int *__restrict__ rowidx, int *rem, int *par,
+                            int *levelItem, int s, int e, int *cl, int *lds) {
+    const int tid = threadIdx.x, len = e - s;
+    int u[kIlp], q0[kIlp], q1[kIlp], vc[kIlp][kE];
+    unsigned kid[kIlp];
+#pragma unroll
+    for (int k = 0; k < kIlp; k++) {
+        const int i = tid + k * kBfsThreads;
+        u[k] = i < len ? ld_wg(&levelItem[s + i]) : -1;
+    }
+#pragma unroll
+    for (int k = 0; k < kIlp; k++) {
+        q0[k] = u[k] >= 0 ? colptr[u[k]] : 0;
+        q1[k] = u[k] >= 0 ? colptr[u[k] + 1] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < kIlp; k++)
+#pragma unroll
+        for (int j = 0; j < kE; j++)
+            vc[k][j] = q0[k] + j < q1[k] ? rowidx[q0[k] + j] : -1;
+    // A: decrement dependants, raise their parent position
+#pragma unroll
+    for (int k = 0; k < kIlp; k++) {
+        const int p = s + tid + k * kBfsThreads;
+#pragma unroll
+        for (int j = 0; j < kE; j++) {
+            const int v = vc[k][j];
+            if (v < 0 || v == u[k]) continue;
+            __hip_atomic_fetch_add(&rem[v], -1, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_max(&par[v], p, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        for (int q = q0[k] + kE; q < q1[k]; q++) {
+            const int v = rowidx[q];
+            if (v == u[k]) continue;
+            __hip_atomic_fetch_add(&rem[v], -1, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_max(&par[v], p, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    // B: children (rem 0, parent p) per node
+#pragma unroll
+    for (int k = 0; k < kIlp; k++) {
+        const int i = tid + k * kBfsThreads;
+        const int p = s + i;
+        int c = 0;
+        unsigned bits = 0;
+#pragma unroll
+        for (int j = 0; j < kE; j++) {
+            const int v = vc[k][j];
+            const bool cand = v >= 0 && v != u[k];
+            const int rv = cand ? ld_wg(&rem[v]) : 1;
+            const int pv = cand ? ld_wg(&par[v]) : -1;
+            if (cand && rv == 0 && pv == p) {
+                c++;
+                bits |= 1u << j;
+            }
+        }
+        for (int q = q0[k] + kE; q < q1[k]; q++) {
+            const int v = rowidx[q];
+            if (v != u[k] && ld_wg(&rem[v]) == 0 && ld_wg(&par[v]) == p) c++;
+        }
+        kid[k] = bits;
+        if (i < len) cl[i] = c;
+    }
+    __syncthreads();
+    const int total = narrow_scan(cl, len, lds);
+    // C: children in column order at the node's offset
+#pragma unroll
+    for (int k = 0; k < kIlp; k++) {
+        const int i = tid + k * kBfsThreads;
+        if (i >= len) continue;
+        const int p = s + i;
+        int o = e + cl[i];
+#pragma unroll
+        for (int j = 0; j < kE; j++)
+            if ((kid[k] >> j) & 1u)
+                __hip_atomic_store(&levelItem[o++], vc[k][j], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        for (int q = q0[k] + kE; q < q1[k]; q++) {
+            const int v = rowidx[q];
+            if (v != u[k] && ld_wg(&rem[v]) == 0 && ld_wg(&par[v]) == p)
+                __hip_atomic_store(&levelItem[o++], v, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    }
+    return total;
+}
+
+// One narrow level of any width up to kNarrow: the same phases, each
+// re-reading the frontier's columns.
+__device__ int level_generic(const int *__restrict__ colptr,
+                             const int *__restrict__ rowidx, int *rem, int *par,
+                             int *levelItem, int s, int e, int *cl, int *lds) {
+    const int tid = threadIdx.x, len = e - s;
+    for (int p = s + tid; p < e; p += blockDim.x) {
+        const int u = ld_wg(&levelItem[p]);
+        for (int q = colptr[u]; q < colptr[u + 1]; q++) {
+            const int v = rowidx[q];
+            if (v == u) continue;
+            __hip_atomic_fetch_add(&rem[v], -1, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_max(&par[v], p, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    for (int i = tid; i < len; i += blockDim.x) {
+        const int p = s + i;
+        const int u = ld_wg(&levelItem[p]);
+        int c = 0;
+        for (int q = colptr[u]; q < colptr[u + 1]; q++) {
+            const int v = rowidx[q];
+            if (v != u && ld_wg(&rem[v]) == 0 && ld_wg(&par[v]) == p) c++;
+        }
+        cl[i] = c;
+    }
+    __syncthreads();
+    const int total = narrow_scan(cl, len, lds);
+    for (int i = tid; i < len; i += blockDim.x) {
+        const int p = s + i;
+        const int u = ld_wg(&levelItem[p]);
+        int o = e + cl[i];
+        for (int q = colptr[u]; q < colptr[u + 1]; q++) {
+            const int v = rowidx[q];
+            if (v != u && ld_wg(&rem[v]) == 0 && ld_wg(&par[v]) == p)
+                __hip_atomic_store(&levelItem[o++], v, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    }
+    return total;
+}
 
 // Consecutive narrow levels inside workgroup 0: the same A / B / scan / C
 // phases, separated by workgroup barriers.  Frontier counts live in LDS.
@@ -133,58 +300,10 @@ __device__ void bfs_narrow(const int *__restrict__ colptr,
     int nl = ld_relaxed(&ctl->nlevel);
     int stalled = 0;
     while (e < m && e - s <= kNarrow) {
-        const int len = e - s;
-        for (int p = s + tid; p < e; p += blockDim.x) {
-            const int u = ld_wg(&levelItem[p]);
-            for (int q = colptr[u]; q < colptr[u + 1]; q++) {
-                const int v = rowidx[q];
-                if (v == u) continue;
-                __hip_atomic_fetch_add(&rem[v], -1, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-                __hip_atomic_fetch_max(&par[v], p, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-            }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __syncthreads();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        for (int i = tid; i < len; i += blockDim.x) {
-            const int p = s + i;
-            const int u = ld_wg(&levelItem[p]);
-            int c = 0;
-            for (int q = colptr[u]; q < colptr[u + 1]; q++) {
-                const int v = rowidx[q];
-                if (v != u && ld_wg(&rem[v]) == 0 &&
-                    ld_wg(&par[v]) == p)
-                    c++;
-            }
-            cl[i] = c;
-        }
-        __syncthreads();
-        const int per = (len + blockDim.x - 1) / blockDim.x;
-        const int lo = min(len, tid * per), hi = min(len, lo + per);
-        int local = 0;
-        for (int i = lo; i < hi; i++) local += cl[i];
-        int total;
-        int run = block_excl_scan(local, lds, &total);
-        for (int i = lo; i < hi; i++) {
-            const int t = cl[i];
-            cl[i] = run;
-            run += t;
-        }
-        __syncthreads();
-        for (int i = tid; i < len; i += blockDim.x) {
-            const int p = s + i;
-            const int u = ld_wg(&levelItem[p]);
-            int o = e + cl[i];
-            for (int q = colptr[u]; q < colptr[u + 1]; q++) {
-                const int v = rowidx[q];
-                if (v != u && ld_wg(&rem[v]) == 0 &&
-                    ld_wg(&par[v]) == p)
-                    __hip_atomic_store(&levelItem[o++], v, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-            }
-        }
+        const int total =
+            e - s <= kIlp * kBfsThreads
+                ? level_cached(colptr, rowidx, rem, par, levelItem, s, e, cl, lds)
+                : level_generic(colptr, rowidx, rem, par, levelItem, s, e, cl, lds);
         if (tid == 0) levelPtr[nl + 1] = e + total;
         nl++;
         s = e;
