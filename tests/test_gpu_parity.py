@@ -165,10 +165,15 @@ def test_handworked_fixture_on_device(oracle, torch):
     assert cp.tolist() == g["colptr_new"] and ri.tolist() == g["rowidx_new"]
 
 
-def test_multi_rhs_and_leading_dimension(oracle, torch):
+@pytest.mark.parametrize("schedule,k", [(0, 3), (1, 3), (2, 3), (2, 40)])
+def test_multi_rhs_and_leading_dimension(oracle, torch, schedule, k):
+    """Columns at a leading dimension past n, through the tuned, blocked
+    and fused schedules.  With at most 4 fused waves, 3 columns are an
+    ordinary launch (at most 12 workgroups) and 40 a cooperative one.  Rows
+    n..ld-1 of x stay untouched."""
     f = gen.banded(5000, seed=8)
-    p = _plan(f, 2)
-    k, ld = 3, f.n + 17
+    p = _plan(f, 2, schedule=schedule, fused_waves=4)
+    ld = f.n + 17
     B = np.random.default_rng(1).uniform(-1, 1, (k, ld))
     Bd = torch.as_tensor(B, device="cuda").t()          # (ld, k) column-major
     X = torch.zeros((k, ld), dtype=torch.float64, device="cuda").t()
