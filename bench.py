@@ -396,7 +396,10 @@ def main():
                          "rows_per_s": round(f.n / (pre_ms * 1e-3), 1),
                          "includes": "level analysis, reorder, partition, "
                                      "blocks, selector, schedule build and "
-                                     "tuning"}
+                                     "tuning",
+                         "phases_ms": {k: round(info.phase_ms[i], 1) for i, k in
+                                       enumerate(("levels", "blocks",
+                                                  "fused_schedule", "tuning"))}}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(f, info.lv, args.cpu_budget)
         out["preprocess"]["cpu_port_ms"] = out["cpu_baseline"]["preprocess_ms"]

@@ -152,8 +152,12 @@ typedef struct {
     int nblocks;          /* 2^(lv+1) - 1 */
     int nlevel;           /* global level count */
     int schedule;         /* schedule a solve uses (RB_SCHED_BLOCKED/FUSED) */
-    double preprocess_ms; /* plan build time */
+    double preprocess_ms; /* plan build time (phases 0-2 below) */
     size_t device_bytes;  /* device memory held by the plan */
+    double phase_ms[4];   /* host-clock ms of the build's phases: level
+                             analysis; reorder, partition, blocks, selector
+                             and storage; fused schedule build; schedule
+                             tuning (timed solves, not in preprocess_ms) */
 } rb_plan_info;
 
 rb_status rb_plan_info_get(const rb_plan *plan, rb_plan_info *info);

@@ -189,7 +189,13 @@ rb_status rb_plan_create(const int *d_colptr, const int *d_rowidx,
     }
     if (st == RB_OK)
         st = build_plan(p, d_colptr, d_rowidx, d_val, (hipStream_t)stream);
-    if (st == RB_OK) st = choose_schedule(p, (hipStream_t)stream);
+    if (st == RB_OK) {
+        auto t0 = std::chrono::steady_clock::now();
+        st = choose_schedule(p, (hipStream_t)stream);
+        p->phase_ms[3] = std::chrono::duration<double, std::milli>(
+                             std::chrono::steady_clock::now() - t0)
+                             .count();
+    }
     if (st != RB_OK) {
         (void)hipStreamSynchronize((hipStream_t)stream);
         destroy_graph(p);
@@ -314,6 +320,7 @@ rb_status rb_plan_info_get(const rb_plan *p, rb_plan_info *info) {
     info->schedule = p->schedule;
     info->preprocess_ms = p->preprocess_ms;
     info->device_bytes = p->arena.bytes;
+    for (int i = 0; i < 4; i++) info->phase_ms[i] = p->phase_ms[i];
     return ok();
 }
 

@@ -106,6 +106,9 @@ struct rb_plan {
     int device = 0;
     int schedule = RB_SCHED_BLOCKED;
     double preprocess_ms = 0.0;
+    // plan build phases (host clock, ms): level analysis, reorder to
+    // selector and storage, fused schedule build, schedule tuning
+    double phase_ms[4] = {0.0, 0.0, 0.0, 0.0};
     rb::Arena arena;
 
     // permutation and global levels

@@ -526,6 +526,7 @@ rb_status build_plan(rb_plan *p, const int *d_colptr, const int *d_rowidx,
     RB_HIP(hipGetLastError());
     RB_TRY(bfs_levels(d_colptr, d_rowidx, rowcnt, m, bfs_item, p->d_levelPtr,
                       &p->nlevel, s));
+    auto t_lv = std::chrono::steady_clock::now();
 
     // ---- 2. level-order permutation of the CSC
     int *newpos, *pcolcnt, *ecol;
@@ -825,8 +826,10 @@ rb_status build_plan(rb_plan *p, const int *d_colptr, const int *d_rowidx,
     }
 
     // ---- 9. fused schedule storage (chain-tiled partition of the rows)
+    auto t_f0 = std::chrono::steady_clock::now();
     if (p->want_schedule != RB_SCHED_BLOCKED && m > 0)
         RB_TRY(build_fused(p, prptr, ccol, cval, p->fused_waves, s));
+    auto t_f1 = std::chrono::steady_clock::now();
 
     // ---- 10. FP32 + FP64-refinement storage: FP32 copies of the slot
     // values, and the level-ordered CSR with original ids (row i is
@@ -867,6 +870,12 @@ rb_status build_plan(rb_plan *p, const int *d_colptr, const int *d_rowidx,
     auto t1 = std::chrono::steady_clock::now();
     p->preprocess_ms =
         std::chrono::duration<double, std::milli>(t1 - t0).count();
+    auto ms = [](auto a, auto b) {
+        return std::chrono::duration<double, std::milli>(b - a).count();
+    };
+    p->phase_ms[0] = ms(t0, t_lv);
+    p->phase_ms[2] = ms(t_f0, t_f1);
+    p->phase_ms[1] = p->preprocess_ms - p->phase_ms[0] - p->phase_ms[2];
     return RB_OK;
 }
 

@@ -54,7 +54,8 @@ class rb_plan_info(ctypes.Structure):
                 ("substitution", ctypes.c_int), ("lv", ctypes.c_int),
                 ("nblocks", ctypes.c_int), ("nlevel", ctypes.c_int),
                 ("schedule", ctypes.c_int), ("preprocess_ms", ctypes.c_double),
-                ("device_bytes", ctypes.c_size_t)]
+                ("device_bytes", ctypes.c_size_t),
+                ("phase_ms", ctypes.c_double * 4)]
 
 
 def build(jobs=8):

@@ -190,6 +190,24 @@ def test_multi_rhs_and_leading_dimension(oracle, torch, schedule, k):
         assert np.all(Xh[j, f.n:] == 0.0)   # padding rows untouched
 
 
+@pytest.mark.parametrize("schedule", [0, 1, 2])
+def test_plan_build_phases(torch, schedule):
+    """The plan reports its build phases: non-negative, the first three
+    adding up to preprocess_ms; timed tuning solves only when the tuner
+    chooses, and no fused build when the blocked schedule is asked for
+    (those phases are then two clock reads apart)."""
+    f = gen.laplacian(60)
+    inf = _plan(f, 2, schedule=schedule).info()
+    ph = list(inf.phase_ms)
+    assert all(t >= 0.0 for t in ph)
+    assert abs(sum(ph[:3]) - inf.preprocess_ms) <= 1e-9 * max(1.0, inf.preprocess_ms)
+    assert ph[0] > 0.0
+    if schedule != 0:
+        assert ph[3] < 0.05
+    if schedule == 1:
+        assert ph[2] < 0.05
+
+
 def test_deterministic_bitwise(torch):
     f = gen.powerlaw(50000, seed=2)
     p = _plan(f, 3)
