@@ -782,6 +782,13 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
                            const std::vector<double> &invd, int Wmax,
                            hipStream_t s) {
     auto t0 = std::chrono::steady_clock::now();
+    // phase times for RB_FUSED_TRACE=1
+    std::vector<std::pair<const char *, double>> marks;
+    auto mark = [&](const char *name) {
+        marks.emplace_back(name, std::chrono::duration<double, std::milli>(
+                                     std::chrono::steady_clock::now() - t0)
+                                     .count());
+    };
     const int m = p->m;
     const bool fwd = p->subst == 0;
     if (Wmax <= 0) {
@@ -892,6 +899,7 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
             i = j;
         }
     }
+    mark("waves+steps");
     // ---- entry classes: DPP (previous step, lane l-1 / l / l+1), ring
     // (same wave, within kR steps), global (everything else)
     auto is_dpp = [&](int q, int u) {
@@ -901,16 +909,58 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
     auto is_loc = [&](int q, int u) {
         return wave[u] == wave[q] && stepw[q] - stepw[u] <= kR;
     };
-    long n_all = 0, n_dpp = 0, n_loc = 0;
-    for (int q = 0; q < m; q++) {
-        const int r = pof(q);
-        for (int e = rp[r]; e < rp[r + 1]; e++) {
-            if (cc[e] == r) continue;
-            n_all++;
-            n_dpp += is_dpp(q, qof(cc[e]));
-            n_loc += is_loc(q, qof(cc[e]));
-        }
+    // one pass over the entries, rows split over threads: counts of all,
+    // DPP and same-wave-window entries, DPP entries by side (lane l-1 or
+    // l+1 of the previous step), and the most window entries of a row
+    // outside / including the DPP ones (which holds depends on the format)
+    struct Tally {
+        long all = 0, dpp = 0, loc = 0, lo = 0, up = 0;
+        int emax_dpp = 0, emax_all = 0;
+    };
+    const int nth_c = std::max(1, std::min({16, (int)std::thread::hardware_concurrency(),
+                                            (m + 65535) / 65536}));
+    std::vector<Tally> tally(nth_c);
+    {
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nth_c; t++)
+            pool.emplace_back([&, t]() {
+                Tally &c = tally[t];
+                const int q0 = (int)((long)m * t / nth_c);
+                const int q1 = (int)((long)m * (t + 1) / nth_c);
+                for (int q = q0; q < q1; q++) {
+                    const int r = pof(q);
+                    int k_dpp = 0, k_all = 0;
+                    for (int e = rp[r]; e < rp[r + 1]; e++) {
+                        if (cc[e] == r) continue;
+                        const int u = qof(cc[e]);
+                        const bool d = is_dpp(q, u), l = is_loc(q, u);
+                        c.all++;
+                        c.dpp += d;
+                        c.loc += l;
+                        if (d) {
+                            c.lo += lanew[u] < lanew[q];
+                            c.up += lanew[u] > lanew[q];
+                        }
+                        k_dpp += l && !d;
+                        k_all += l;
+                    }
+                    c.emax_dpp = std::max(c.emax_dpp, k_dpp);
+                    c.emax_all = std::max(c.emax_all, k_all);
+                }
+            });
+        for (auto &t : pool) t.join();
     }
+    Tally tot;
+    for (const Tally &c : tally) {
+        tot.all += c.all;
+        tot.dpp += c.dpp;
+        tot.loc += c.loc;
+        tot.lo += c.lo;
+        tot.up += c.up;
+        tot.emax_dpp = std::max(tot.emax_dpp, c.emax_dpp);
+        tot.emax_all = std::max(tot.emax_all, c.emax_all);
+    }
+    const long n_all = tot.all, n_dpp = tot.dpp, n_loc = tot.loc;
     // A schedule whose entries are mostly cross-wave runs at the resolver's
     // pace and loses to the blocked schedule on one right-hand side (the
     // tuner's measurements on banded and power-law factors: 3-12x slower).
@@ -920,34 +970,13 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
     if (p->want_schedule == RB_SCHED_AUTO && p->tune_rhs <= 1 &&
         4 * (n_all - n_loc) > n_all)
         return RB_OK;
-    // DPP entries by side: lane l-1 (lo) or l+1 (up) of the previous step
-    long n_lo = 0, n_up = 0;
-    for (int q = 0; q < m; q++) {
-        const int r = pof(q);
-        for (int e = rp[r]; e < rp[r + 1]; e++) {
-            if (cc[e] == r) continue;
-            const int u = qof(cc[e]);
-            if (is_dpp(q, u)) {
-                n_lo += lanew[u] < lanew[q];
-                n_up += lanew[u] > lanew[q];
-            }
-        }
-    }
+    const long n_lo = tot.lo, n_up = tot.up;
     const int NDPP = !(n_dpp > 0 && 4 * n_dpp >= n_all) ? 0
                    : n_up == 0 ? 1 : n_lo == 0 ? 2 : 3;
-    int emax = 0;
-    for (int q = 0; q < m; q++) {
-        const int r = pof(q);
-        int k = 0;
-        for (int e = rp[r]; e < rp[r + 1]; e++) {
-            if (cc[e] == r) continue;
-            const int u = qof(cc[e]);
-            if (is_loc(q, u) && !(NDPP && is_dpp(q, u))) k++;
-        }
-        emax = std::max(emax, k);
-    }
+    const int emax = NDPP ? tot.emax_dpp : tot.emax_all;
     const int NRING = ring_class(std::min(emax, 8));
     const int K = words_for(NDPP, NRING);
+    mark("classes");
     // pad every wave to whole prefetch blocks with empty steps
     {
         const int Bp = block_steps(K);
@@ -1055,6 +1084,7 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
             });
         for (auto &t : pool) t.join();
     }
+    mark("emit");
     // every tile's slots listed by ascending row, padding lanes last
     const size_t nslot_all = (size_t)NS * 64;
     const int ntile = (int)((nslot_all + kTile - 1) / kTile);
@@ -1088,6 +1118,7 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
             });
         for (auto &t : pool) t.join();
     }
+    mark("tiles");
     std::vector<int> gsrc, gmeta;
     std::vector<double> gval;
     std::vector<char> needpub(m, 0);
@@ -1125,6 +1156,7 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
         gmeta.push_back(1);
     }
     if (pub.empty()) pub.push_back(0);
+    mark("join+pub");
     // ---- upload
     std::unique_ptr<FusedPlan> owner(new FusedPlan());
     FusedPlan *F = owner.get();
@@ -1175,6 +1207,12 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
     RB_HIP(hipStreamSynchronize(s));
     auto t1 = std::chrono::steady_clock::now();
     F->build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    mark("upload");
+    if (getenv("RB_FUSED_TRACE")) {
+        fprintf(stderr, "fused build (cumulative ms):");
+        for (auto &mk : marks) fprintf(stderr, " %s %.1f", mk.first, mk.second);
+        fprintf(stderr, "\n");
+    }
     device_shape(F);
     delete p->fused;
     p->fused = owner.release();
@@ -1186,6 +1224,7 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
 rb_status build_fused(rb_plan *p, const int *d_prptr, const int *d_ccol,
                       const double *d_cval, int Wmax, hipStream_t s) {
     const int m = p->m, nnz = p->nnz;
+    auto tp = std::chrono::steady_clock::now();
     std::vector<int> rp(m + 1), cc(nnz > 0 ? nnz : 1), lp(p->nlevel + 1),
         lf(m > 0 ? m : 1);
     std::vector<double> cv(nnz > 0 ? nnz : 1), invd(m > 0 ? m : 1);
@@ -1207,6 +1246,11 @@ rb_status build_fused(rb_plan *p, const int *d_prptr, const int *d_ccol,
                               hipMemcpyDeviceToHost, s));
     }
     RB_HIP(hipStreamSynchronize(s));
+    if (getenv("RB_FUSED_TRACE"))
+        fprintf(stderr, "fused build: host copies of the tables %.1f ms\n",
+                std::chrono::duration<double, std::milli>(
+                    std::chrono::steady_clock::now() - tp)
+                    .count());
     return build_fused_host(p, rp, cc, cv, lp, lf, invd, Wmax, s);
 }
 
