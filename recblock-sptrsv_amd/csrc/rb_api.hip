@@ -68,21 +68,33 @@ static rb_status choose_schedule(rb_plan *p, hipStream_t s) {
     (void)hipEventCreate(&e1);
     // the work of a solve does not depend on the values: b = 0
     if (hipMemsetAsync(db, 0, bytes, s) != hipSuccess) st = RB_ERR_HIP;
+    // fused first; the blocked schedule's first timed solve settles the
+    // choice when it is more than twice the fused time (its other two are
+    // then not run)
     double ms[2] = {0.0, 0.0};
-    for (int c = 0; c < 2 && st == RB_OK; c++) {
+    for (int c = 1; c >= 0 && st == RB_OK; c--) {
         auto run = [&]() {
             return c == 0 ? solve_blocked(p, db, m, dx, m, k, s)
                           : solve_fused(p, db, m, dx, m, k, s);
         };
         st = run();  // warm
         if (st != RB_OK) break;
-        (void)hipEventRecord(e0, s);
-        for (int r = 0; r < 3 && st == RB_OK; r++) st = run();
-        (void)hipEventRecord(e1, s);
-        (void)hipEventSynchronize(e1);
-        float t = 0.f;
-        (void)hipEventElapsedTime(&t, e0, e1);
-        ms[c] = t / 3.0;
+        // timed solves: fused 3 at once; blocked 1, then 2 more if needed
+        auto timed = [&](int reps) {
+            float t = 0.f;
+            (void)hipEventRecord(e0, s);
+            for (int r = 0; r < reps && st == RB_OK; r++) st = run();
+            (void)hipEventRecord(e1, s);
+            (void)hipEventSynchronize(e1);
+            (void)hipEventElapsedTime(&t, e0, e1);
+            return (double)t;
+        };
+        if (c == 1) {
+            ms[1] = timed(3) / 3.0;
+        } else {
+            const double t1 = timed(1);
+            ms[0] = t1 > 2.0 * ms[1] || st != RB_OK ? t1 : (t1 + timed(2)) / 3.0;
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
