@@ -804,14 +804,26 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
     // ---- wave assignment (chain following) and steps
     std::vector<int> wave(m), stepw(m), lanew(m);
     std::vector<int> par(m, -1);
-    for (int q = 0; q < m; q++) {
-        const int r = pof(q);
-        int best = -1;
-        for (int k = rp[r]; k < rp[r + 1]; k++) {
-            if (cc[k] == r) continue;
-            best = std::max(best, qof(cc[k]));
-        }
-        par[q] = best;
+    // a row's parent: its latest dependency (rows split over threads)
+    const int nth_h = std::max(1, std::min({16, (int)std::thread::hardware_concurrency(),
+                                            (m + 65535) / 65536}));
+    {
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nth_h; t++)
+            pool.emplace_back([&, t]() {
+                const int q0 = (int)((long)m * t / nth_h);
+                const int q1 = (int)((long)m * (t + 1) / nth_h);
+                for (int q = q0; q < q1; q++) {
+                    const int r = pof(q);
+                    int best = -1;
+                    for (int k = rp[r]; k < rp[r + 1]; k++) {
+                        if (cc[k] == r) continue;
+                        best = std::max(best, qof(cc[k]));
+                    }
+                    par[q] = best;
+                }
+            });
+        for (auto &t : pool) t.join();
     }
     int W = 0;
     std::vector<int> cnt;           // per wave count at current level
@@ -861,7 +873,9 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
     for (int L = 0; L < nlev; L++)
         for (int q = lp[L]; q < lp[L + 1]; q++) lvl[q] = L;
     std::vector<std::vector<std::array<int, 64>>> steps(W);
-    for (int w = 0; w < W; w++) {
+    // waves are independent here (a lane follows a parent of its own
+    // wave): one thread per wave
+    auto make_steps = [&](int w) {
         auto &rows = wave_rows[w];
         size_t i = 0;
         while (i < rows.size()) {
@@ -898,6 +912,16 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
             }
             i = j;
         }
+    };
+    {
+        const int nth = std::max(1, std::min({W, 16, (int)std::thread::hardware_concurrency()}));
+        std::vector<std::thread> pool;
+        std::atomic<int> next{0};
+        for (int t = 0; t < nth; t++)
+            pool.emplace_back([&]() {
+                for (int w; (w = next.fetch_add(1)) < W;) make_steps(w);
+            });
+        for (auto &t : pool) t.join();
     }
     mark("waves+steps");
     // ---- entry classes: DPP (previous step, lane l-1 / l / l+1), ring
