@@ -729,6 +729,27 @@ struct FusedPlan {
     ~FusedPlan();
 };
 
+// std::vector storage whose elements are default-initialised (left
+// unset for ints and doubles) when the vector is sized
+template <class T>
+struct NoInit : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = NoInit<U>;
+    };
+    NoInit() = default;
+    template <class U>
+    NoInit(const NoInit<U> &) {}
+    template <class U>
+    void construct(U *p) {
+        ::new ((void *)p) U;
+    }
+    template <class U, class... A>
+    void construct(U *p, A &&...a) {
+        ::new ((void *)p) U(std::forward<A>(a)...);
+    }
+};
+
 static int ring_class(int e) { return e == 0 ? 0 : (e <= 2 ? 2 : (e <= 4 ? 4 : 8)); }
 
 static int words_for(int ndpp, int nring) { return words_of(ndpp, nring); }
@@ -1017,9 +1038,11 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
     auto slot_of = [&](int q) {
         return (step0[wave[q]] + stepw[q]) * 64 + lanew[q];
     };
-    std::vector<int4> rec((size_t)NS * K * 64);
-    std::vector<int> origv((size_t)NS * 64, -1);
-    std::vector<double> sinvd((size_t)NS * 64, 0.0);
+    // every element of these is written by the emitting threads: no
+    // serial zero-fill of ~100 MB before they start
+    std::vector<int4, NoInit<int4>> rec((size_t)NS * K * 64);
+    std::vector<int, NoInit<int>> origv((size_t)NS * 64);
+    std::vector<double, NoInit<double>> sinvd((size_t)NS * 64);
     const int NB = block_steps(K), FF = fifo_groups(K);
     // Per wave, in parallel on host threads (waves write disjoint record,
     // slot and 1/diag ranges; their global entries, sources and block
@@ -1085,6 +1108,9 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
                         gord++;
                     }
                     sinvd[(size_t)(step0[w] + si) * 64 + l] = invd[r];
+                } else {  // padding lane
+                    origv[(size_t)(step0[w] + si) * 64 + l] = -1;
+                    sinvd[(size_t)(step0[w] + si) * 64 + l] = 0.0;
                 }
                 f[0] = hasg ? has_group_sel(FF) : 0;
                 f[1] = hasg ? gofs * 8 : no_group_off(FF);
