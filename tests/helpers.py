@@ -105,3 +105,50 @@ def rel_err(x, xref):
 def residual(f, x, b):
     r = np.asarray(b) - f.matvec(np.asarray(x))
     return float(np.linalg.norm(r) / max(np.linalg.norm(b), 1e-300))
+
+
+def selector_factor(case):
+    """Lower-triangular factor (lv = 1 layout, see tests/golden/selector.json)
+    for one hand-worked selector case:
+      fan_chain(R, K): rows 0..R-1 have no dependency; chain rows R..R+K-1,
+          the first depending on every root, each later one on every root
+          and on the chain row before it; then s = R + K rows that each
+          depend on the last chain row (they form block tri1);
+      chain(n): row i depends on row i-1 for 0 < i < n; then n rows that
+          each depend on row n-1;
+      long_chain(n): row i depends on row i-1 for every 0 < i < n;
+      square(s, lens): rows 0..s-1 have no dependency; row s + a depends on
+          columns 0..lens[a]-1; the remaining s - len(lens) rows each depend
+          on row s."""
+    k = case["kind"]
+    deps = {}
+    if k == "fan_chain":
+        R, K = case["R"], case["K"]
+        for j in range(K):
+            row = R + j
+            deps[row] = list(range(R)) + ([row - 1] if j > 0 else [])
+        s = R + K
+        for i in range(s, 2 * s):
+            deps[i] = [s - 1]
+        n = 2 * s
+    elif k == "chain":
+        c = case["n"]
+        for i in range(1, c):
+            deps[i] = [i - 1]
+        for i in range(c, 2 * c):
+            deps[i] = [c - 1]
+        n = 2 * c
+    elif k == "long_chain":
+        n = case["n"]
+        for i in range(1, n):
+            deps[i] = [i - 1]
+    elif k == "square":
+        s, lens = case["s"], case["lens"]
+        for a, L in enumerate(lens):
+            deps[s + a] = list(range(L))
+        for i in range(s + len(lens), 2 * s):
+            deps[i] = [s]
+        n = 2 * s
+    else:
+        raise ValueError(k)
+    return csc_from_deps(n, deps, seed=7)
