@@ -66,7 +66,13 @@ typedef struct {
                           RB_ERR_BAD_STRUCTURE for such input, as before.
                           No counterpart in the reference, which needs
                           sorted, unique row ids. */
-    int reserved[1];
+    int watchdog_spins; /* 0 = default bound (about 4M polls, seconds); > 0:
+                          every bounded wait of a solve gives up after this
+                          many polls.  A small value lets a test drive the
+                          failure path (x all NaN, rb_plan_check reporting
+                          RB_ERR_INTERNAL) while every kernel exits
+                          normally.  No counterpart in the reference, whose
+                          waits are unbounded. */
 } rb_opts;
 
 typedef struct rb_plan rb_plan;
@@ -141,9 +147,12 @@ rb_status rb_solve(rb_plan *plan, const double *d_b, int ldb, double *d_x,
 rb_status rb_plan_destroy(rb_plan *plan);
 
 /* Synchronise stream and report the device-side watchdog: RB_OK, or
- * RB_ERR_INTERNAL when a solve's bounded spin-wait gave up (it then wrote
- * NaN instead of hanging).  No counterpart in the reference, whose spin
- * loops have no bound. */
+ * RB_ERR_INTERNAL when a solve's bounded spin-wait gave up.  From the solve
+ * whose wait gave up until this call, every solve on the plan writes quiet
+ * NaN to all of x (never a finite wrong answer); this call then clears the
+ * flag and re-arms the plan's per-solve state, so later solves run
+ * normally.  No counterpart in the reference, whose spin loops have no
+ * bound. */
 rb_status rb_plan_check(rb_plan *plan, void *stream);
 
 /* Plan summary for inspection and tests. */

@@ -113,6 +113,7 @@ static rb_status choose_schedule(rb_plan *p, hipStream_t s) {
     if (dstat != 0) {
         int z = 0;
         (void)hipMemcpy(p->d_status, &z, sizeof(int), hipMemcpyHostToDevice);
+        fused_rearm(p);
     }
     return RB_OK;
 }
@@ -180,6 +181,7 @@ rb_status rb_plan_create(const int *d_colptr, const int *d_rowidx,
     p->graph_failed = o.no_graph != 0;  // eager launches only
     p->mixed = o.mixed != 0;
     p->tune_rhs = o.tune_rhs;
+    if (o.watchdog_spins > 0) p->spin_limit = (unsigned)o.watchdog_spins;
     rb_status st = RB_OK;
     Arena canon;  // the canonical copy lives until the plan is built
     struct CanonGuard {
@@ -246,9 +248,14 @@ rb_status rb_plan_check(rb_plan *p, void *stream) {
     int st = 0;
     RB_HIP(hipMemcpy(&st, p->d_status, sizeof(int), hipMemcpyDeviceToHost));
     if (st != 0) {
+        // clear the flag and re-arm the per-solve state (an aborted fused
+        // solve leaves slots of both x buffers un-armed)
+        const int z = 0;
+        RB_HIP(hipMemcpy(p->d_status, &z, sizeof(int), hipMemcpyHostToDevice));
+        fused_rearm(p);
         set_error(RB_ERR_INTERNAL,
                   "device watchdog: a solve waited on a value that never "
-                  "arrived (x holds NaN for the affected rows)");
+                  "arrived (x was written as NaN by every solve since)");
         return RB_ERR_INTERNAL;
     }
     return ok();

@@ -55,7 +55,6 @@ namespace {
 
 constexpr int kR = 32;          // LDS x ring depth (steps), a power of two
 constexpr int kU = 2;           // resolver: batches of 64 polled together
-constexpr unsigned kSpin = 1u << 22;
 constexpr int kZeroOff = kR * 64 * 8;  // byte offset of the ring's zero
 constexpr int kWG = 128;        // consumer wave + resolver wave
 
@@ -194,6 +193,7 @@ struct FusedArgs {
     long nslots;         // nsteps * 64: stride between right-hand sides
     int W;               // consumer waves per right-hand side
     int *status;
+    unsigned spin;       // polls before a bounded wait gives up
     // optional trace: per workgroup 16 counters of wall-clock ticks
     // (10 ns): [1] consumer waits, [2] consumer total, [3] steps,
     // [7] resolver FIFO-space wait, [8] resolver poll, [9] batches,
@@ -320,7 +320,7 @@ struct Consumer {
         unsigned spins = 0;
 #pragma nounroll
         while ((fifo_avail = lds_load_u(&sh.ready)) < need) {
-            if (++spins > kSpin || lds_load_u(&sh.abort)) {
+            if (++spins > a.spin || lds_load_u(&sh.abort)) {
                 if (ok && lane == 0) {
                     atomicOr(a.status, 2);
                     lds_store(&sh.abort, 1);
@@ -480,7 +480,7 @@ __device__ __forceinline__ void resolve(const FusedArgs &a, int w, int r,
             unsigned spins = 0;
 #pragma nounroll
             while (gmax - lds_load_u(&sh.cons) >= FF) {
-                if (++spins > kSpin || lds_load_u(&sh.abort)) {
+                if (++spins > a.spin || lds_load_u(&sh.abort)) {
                     abort = true;
                     break;
                 }
@@ -558,7 +558,7 @@ __device__ __forceinline__ void resolve(const FusedArgs &a, int w, int r,
                 if (fp >= n) break;
                 nround++;
                 ++spins;
-                if (spins > kSpin || lds_load_u(&sh.abort) ||
+                if (spins > a.spin || lds_load_u(&sh.abort) ||
                     ((spins & 1023) == 0 && status_now(a.status))) {
                     if (lane == 0) {
                         atomicOr(a.status, 4);
@@ -672,15 +672,18 @@ __global__ void __launch_bounds__(kTileThreads)
 __global__ void __launch_bounds__(kTileThreads)
     k_fused_epilogue(const int *__restrict__ trow,
                      const unsigned short *__restrict__ tperm, long nslots,
-                     const double *__restrict__ xq, double *x, long ldx) {
+                     const double *__restrict__ xq, double *x, long ldx,
+                     const int *status) {
     __shared__ double buf[kTile];
     const long base = (long)blockIdx.x * kTile;
     const int cnt = (int)min((long)kTile, nslots - base);
     const double *in = xq + blockIdx.y * nslots + base;
+    // after a watchdog trip every x of the plan's solves is a quiet NaN
+    const bool tripped = *status != 0;
 #pragma unroll
     for (int k = 0; k < kTile / kTileThreads; k++) {
         const int i = k * kTileThreads + threadIdx.x;
-        if (i < cnt) buf[i] = in[i];
+        if (i < cnt) buf[i] = tripped ? __builtin_nan("") : in[i];
     }
     __syncthreads();
     double *xr = x + blockIdx.y * ldx;
@@ -1440,7 +1443,8 @@ rb_status solve_fused(rb_plan *p, const double *d_b, int ldb, double *d_x,
         RB_HIP(hipGetLastError());
         FusedArgs args{F->d_step0, F->d_gent0, F->d_pub0, F->d_rec,
                        F->d_gsrc, F->d_gval, F->d_gmeta, F->d_pub, cur, nxt,
-                       F->d_brec, (long)nslots, F->W, p->d_status, F->d_trace};
+                       F->d_brec, (long)nslots, F->W, p->d_status,
+                       p->spin_limit, F->d_trace};
         void *kargs[] = {(void *)&args};
         mark_start(p, s);
         const bool ordinary = F->W * nr <= small_grid(F->ncu);
@@ -1455,7 +1459,8 @@ rb_status solve_fused(rb_plan *p, const double *d_b, int ldb, double *d_x,
                                              : "hipLaunchCooperativeKernel(k_fused)");
         mark_end(p, s);
         k_fused_epilogue<<<tiles, kTileThreads, 0, s>>>(
-            F->d_trow, F->d_tperm, (long)nslots, cur, d_x + (size_t)k0 * ldx, ldx);
+            F->d_trow, F->d_tperm, (long)nslots, cur, d_x + (size_t)k0 * ldx, ldx,
+            p->d_status);
         RB_HIP(hipGetLastError());
         for (int r = 0; r < nr; r++) {
             F->armed[cb][r] = 0;
@@ -1464,6 +1469,13 @@ rb_status solve_fused(rb_plan *p, const double *d_b, int ldb, double *d_x,
         F->epoch++;
     }
     return RB_OK;
+}
+
+void fused_rearm(rb_plan *p) {
+    FusedPlan *F = p->fused;
+    if (!F) return;
+    for (int b = 0; b < 2; b++)
+        for (size_t r = 0; r < F->armed[b].size(); r++) F->armed[b][r] = 0;
 }
 
 void fused_destroy(rb_plan *p) {

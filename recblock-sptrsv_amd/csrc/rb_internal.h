@@ -139,6 +139,7 @@ struct rb_plan {
     double *d_xperm = nullptr;  // m
     int *d_tickets = nullptr;   // one per triangular block
     int *d_status = nullptr;    // device watchdog flags (0 = healthy)
+    unsigned spin_limit = 1u << 22;  // polls before a bounded wait gives up
     int scratch_rhs = 0;
 
     // fused schedule (built when requested or for the tuner)
@@ -194,5 +195,8 @@ inline void mark_end(rb_plan *p, hipStream_t s) {
     if (p->timing) (void)hipEventRecord(p->ev1, s);
 }
 void destroy_graph(rb_plan *p);
+// after a watchdog trip: every slot-order x buffer of the fused schedule is
+// filled with the sentinel again before its next use
+void fused_rearm(rb_plan *p);
 int fused_max_waves();
 }  // namespace rb

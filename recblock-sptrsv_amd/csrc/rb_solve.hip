@@ -31,7 +31,6 @@ namespace rb {
 namespace {
 
 constexpr int kT = 256;
-constexpr unsigned kSpinLimit = 1u << 22;  // about seconds of polling
 
 // value type <-> same-width unsigned bits, and the type's sentinel
 template <class T> struct Bits;
@@ -77,11 +76,15 @@ __global__ void k_prologue(const T *__restrict__ b, const int *__restrict__ lf,
     if (i < ntri) tickets[i] = 0;
 }
 
+// x back to the caller's order; after a watchdog trip (status set) every
+// entry is a quiet NaN, so no solve of a tripped plan hands out a finite x
 template <class T>
 __global__ void k_epilogue(const T *__restrict__ xperm,
-                           const int *__restrict__ lf, int m, T *x) {
+                           const int *__restrict__ lf, int m, T *x,
+                           const int *status) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < m) x[lf[i]] = xperm[i];
+    if (i < m)
+        x[lf[i]] = *status ? __builtin_bit_cast(T, Bits<T>::kQNaN) : xperm[i];
 }
 
 template <class T>
@@ -149,7 +152,8 @@ __global__ void __launch_bounds__(kT)
 k_tri_syncfree(const int *__restrict__ rowptr, const int *__restrict__ col,
                const T *__restrict__ val, const T *__restrict__ invd,
                const T *__restrict__ bperm, int sb, int up, int bm,
-               int subst, int *ticket, T *xperm, int *status) {
+               int subst, int *ticket, T *xperm, int *status,
+               unsigned spin_limit) {
     const int lane = threadIdx.x & 63;
     int tile = 0;
     if (lane == 0) tile = atomicAdd(ticket, 1);
@@ -185,7 +189,7 @@ k_tri_syncfree(const int *__restrict__ rowptr, const int *__restrict__ col,
         if (__any(pend)) {
             // watchdog: a dependency that never arrives means corrupt
             // storage; flag it and release the row instead of hanging
-            if (++spins > kSpinLimit) {
+            if (++spins > spin_limit) {
                 if (pend) {
                     atomicOr(status, 1);
                     st_x(&xperm[up + r], __builtin_bit_cast(T, Bits<T>::kQNaN));
@@ -420,7 +424,7 @@ static void launch_blocks(rb_plan *p, const T *val, const T *invd, T *bperm,
                 k_tri_syncfree<T><<<nblk((long)B.blk_m, 64), 64, 0, s>>>(
                     p->d_rowptr, p->d_col, val, invd, bperm, B.slot_base,
                     B.up, B.blk_m, p->subst, p->d_tickets + i / 2, xperm,
-                    p->d_status);
+                    p->d_status, p->spin_limit);
                 break;
             }
         } else {
@@ -517,7 +521,8 @@ static rb_status walk(rb_plan *p, void **exec, const T *val, const T *invd,
     else
         launch_blocks<T>(p, val, invd, bperm, xperm, s);
     mark_end(p, s);
-    k_epilogue<T><<<nblk(m), kT, 0, s>>>(xperm, p->d_levelItem, m, x);
+    k_epilogue<T><<<nblk(m), kT, 0, s>>>(xperm, p->d_levelItem, m, x,
+                                         p->d_status);
     RB_HIP(hipGetLastError());
     return RB_OK;
 }

@@ -9,7 +9,10 @@ import os
 import subprocess
 
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(PKG_ROOT, "librecblock.so")
+# RECBLOCK_LIB names another build of the library to load (a variant built
+# with different compile-time settings); the in-tree build by default
+LIB_PATH = os.environ.get("RECBLOCK_LIB") or os.path.join(PKG_ROOT,
+                                                          "librecblock.so")
 CSRC = os.path.join(PKG_ROOT, "csrc")
 
 # Every exported symbol of include/recblock.h.
@@ -46,7 +49,7 @@ class rb_opts(ctypes.Structure):
                 ("fused_waves", ctypes.c_int), ("no_graph", ctypes.c_int),
                 ("mixed", ctypes.c_int), ("tune_rhs", ctypes.c_int),
                 ("canonicalize", ctypes.c_int),
-                ("reserved", ctypes.c_int * 1)]
+                ("watchdog_spins", ctypes.c_int)]
 
 
 class rb_plan_info(ctypes.Structure):

@@ -7,6 +7,7 @@ interface.  Device memory and streams come from torch (ROCm build); the
 compute is librecblock.so.
 """
 import ctypes
+import sys
 
 import numpy as np
 
@@ -21,18 +22,63 @@ def _torch():
     return torch
 
 
-def _dev_i32(a):
-    torch = _torch()
-    return torch.as_tensor(np.ascontiguousarray(a, np.int32), device="cuda")
-
-
-def _dev_f64(a):
-    torch = _torch()
-    return torch.as_tensor(np.ascontiguousarray(a, np.float64), device="cuda")
-
-
 def _ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _dev_vec(t, dtype, n=None, what="array"):
+    """A 1-D, contiguous device tensor of ``dtype`` holding ``t`` (a host
+    array or a torch tensor on any device).  Tensors already in that form
+    pass through; others are copied.  The C API reads raw device pointers,
+    so a host pointer or a wrong element width must never reach it."""
+    torch = _torch()
+    if not torch.is_tensor(t):
+        np_t = np.int32 if dtype == torch.int32 else np.float64
+        t = torch.as_tensor(np.ascontiguousarray(t, np_t), device="cuda")
+    if t.dim() != 1:
+        raise ValueError("%s must be one-dimensional" % what)
+    if not t.is_cuda or t.dtype != dtype or t.stride(0) != 1:
+        t = t.to(device="cuda", dtype=dtype).contiguous()
+    if n is not None and t.numel() != n:
+        raise ValueError("%s has %d entries, expected %d" % (what, t.numel(), n))
+    return t
+
+
+def _rhs(torch, b, m, what="b"):
+    """(tensor, ld, k) of right-hand sides b: shape (m,) or column-major
+    (m, k), float64, on the device.  b is copied into that form when it is
+    not (a host array, another dtype or device, a row-major block)."""
+    if not torch.is_tensor(b):
+        b = torch.as_tensor(np.asarray(b, np.float64), device="cuda")
+    if b.dim() not in (1, 2) or b.shape[0] != m:
+        raise ValueError("%s must have shape (%d,) or (%d, k)" % (what, m, m))
+    if b.dim() == 1:
+        if not b.is_cuda or b.dtype != torch.float64 or b.stride(0) != 1:
+            b = b.to(device="cuda", dtype=torch.float64).contiguous()
+        return b, m, 1
+    k = b.shape[1]
+    if (not b.is_cuda or b.dtype != torch.float64 or b.stride(0) != 1 or
+            (k > 1 and b.stride(1) < m)):
+        b = b.to(device="cuda", dtype=torch.float64).t().contiguous().t()
+    return b, (b.stride(1) if k > 1 else m), k
+
+
+def _out(torch, x, m, k, like_2d):
+    """The output tensor for k solutions of length m: a new column-major
+    block, or the caller's x after checking it can be written in place."""
+    if x is None:
+        if not like_2d:
+            return torch.empty(m, dtype=torch.float64, device="cuda"), m
+        return torch.empty((k, m), dtype=torch.float64, device="cuda").t(), m
+    if not torch.is_tensor(x) or not x.is_cuda or x.dtype != torch.float64:
+        raise ValueError("x must be a float64 CUDA tensor")
+    if x.shape[0] != m or x.stride(0) != 1 or (x.dim() == 2 and x.shape[1] != k) \
+            or x.dim() not in (1, 2) or (x.dim() == 1 and k != 1):
+        raise ValueError("x must be (m,) or column-major (m, k) matching b")
+    ld = x.stride(1) if x.dim() == 2 and k > 1 else m
+    if x.dim() == 2 and k > 1 and ld < m:
+        raise ValueError("x columns overlap")
+    return x, ld
 
 
 def _stream_ptr(stream):
@@ -51,14 +97,16 @@ class Plan:
     def __init__(self, colptr, rowidx, val, substitution=0, lv=-1,
                  schedule=_lib.SCHED_AUTO, stream=None, device_id=0,
                  fused_waves=0, graph=True, mixed=False, tune_rhs=1,
-                 canonicalize=False):
+                 canonicalize=False, watchdog_spins=0):
         torch = _torch()
         self._keep = []
-        dcp = colptr if torch.is_tensor(colptr) else _dev_i32(colptr)
-        dri = rowidx if torch.is_tensor(rowidx) else _dev_i32(rowidx)
-        dv = val if torch.is_tensor(val) else _dev_f64(val)
+        dcp = _dev_vec(colptr, torch.int32, what="colptr")
         self.m = int(dcp.numel()) - 1
+        if self.m < 0:
+            raise ValueError("colptr needs m + 1 >= 1 entries")
+        dri = _dev_vec(rowidx, torch.int32, what="rowidx")
         self.nnz = int(dri.numel())
+        dv = _dev_vec(val, torch.float64, self.nnz, what="val")
         opts = _lib.rb_opts()
         lib().rb_opts_default(ctypes.byref(opts))
         opts.schedule = schedule
@@ -68,6 +116,7 @@ class Plan:
         opts.mixed = 1 if mixed else 0
         opts.tune_rhs = tune_rhs
         opts.canonicalize = 1 if canonicalize else 0
+        opts.watchdog_spins = int(watchdog_spins)
         h = ctypes.c_void_p()
         check(lib().rb_plan_create(_ptr(dcp), _ptr(dri), _ptr(dv), self.m,
                                    self.m, self.nnz, substitution, lv,
@@ -91,6 +140,10 @@ class Plan:
             self._h = None
 
     def __del__(self):
+        # at interpreter exit the HIP runtime (and a profiler attached to
+        # it) may already be finalised: leave the memory to the process end
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -103,49 +156,23 @@ class Plan:
 
     # ------------------------------------------------------------ solve
     def solve(self, b, x=None, stream=None):
-        """Solve for b (torch device tensor, shape (m,) or (m, k) with
-        column-major storage, i.e. b.t() contiguous).  Returns x."""
+        """Solve for b: shape (m,) or (m, k), column-major on the device
+        (b.t() contiguous); other forms are copied first.  Returns x (the
+        caller's x when given: float64, on the device, same shape)."""
         torch = _torch()
-        if not torch.is_tensor(b):
-            b = _dev_f64(b)
-        if b.dim() == 1:
-            nrhs, ldb = 1, self.m
-            if x is None:
-                x = torch.empty_like(b)
-            ldx = self.m
-        else:
-            nrhs = b.shape[1]
-            if b.stride(0) != 1:
-                raise ValueError("b must be column-major (b.t() contiguous)")
-            ldb = b.stride(1)
-            if x is None:
-                x = torch.empty((nrhs, self.m), dtype=b.dtype,
-                                device=b.device).t()
-            ldx = x.stride(1)
+        b, ldb, nrhs = _rhs(torch, b, self.m)
+        x, ldx = _out(torch, x, self.m, nrhs, b.dim() == 2)
         check(lib().rb_solve(self._h, _ptr(b), ldb, _ptr(x), ldx, nrhs,
                              _stream_ptr(stream)))
         return x
 
     def solve_f32ir(self, b, x=None, tol=1e-12, max_iter=10, stream=None):
         """FP32 solves with FP64 refinement (plan built with mixed=True).
-        b as in solve().  Returns (x, iterations, relative residual)."""
+        b and x as in solve().  Returns (x, iterations, relative
+        residual)."""
         torch = _torch()
-        if not torch.is_tensor(b):
-            b = _dev_f64(b)
-        if b.dim() == 1:
-            nrhs, ldb = 1, self.m
-            if x is None:
-                x = torch.empty_like(b)
-            ldx = self.m
-        else:
-            nrhs = b.shape[1]
-            if b.stride(0) != 1:
-                raise ValueError("b must be column-major (b.t() contiguous)")
-            ldb = b.stride(1)
-            if x is None:
-                x = torch.empty((nrhs, self.m), dtype=b.dtype,
-                                device=b.device).t()
-            ldx = x.stride(1)
+        b, ldb, nrhs = _rhs(torch, b, self.m)
+        x, ldx = _out(torch, x, self.m, nrhs, b.dim() == 2)
         it = ctypes.c_int(0)
         rr = ctypes.c_double(0.0)
         check(lib().rb_solve_f32ir(self._h, _ptr(b), ldb, _ptr(x), ldx, nrhs,
@@ -229,15 +256,6 @@ class Plan:
         return cp, ri[:self.nnz], vv[:self.nnz]
 
 
-def _colmajor(torch, b, m):
-    """(pointer, ld, k) of a (m,) or column-major (m, k) device tensor."""
-    if b.dim() == 1:
-        return _ptr(b), m, 1
-    if b.stride(0) != 1:
-        raise ValueError("right-hand sides must be column-major (b.t() contiguous)")
-    return _ptr(b), b.stride(1), b.shape[1]
-
-
 class Batch:
     """Many independent factors preprocessed as one block-diagonal plan and
     solved together (rb_batch_*).  ``factors``: objects with ``colptr``,
@@ -255,9 +273,9 @@ class Batch:
         self._keep = []
         cps, ris, vs, ms, zs = [], [], [], [], []
         for f in factors:
-            cp = f.colptr if torch.is_tensor(f.colptr) else _dev_i32(f.colptr)
-            ri = f.rowidx if torch.is_tensor(f.rowidx) else _dev_i32(f.rowidx)
-            v = f.val if torch.is_tensor(f.val) else _dev_f64(f.val)
+            cp = _dev_vec(f.colptr, torch.int32, what="colptr")
+            ri = _dev_vec(f.rowidx, torch.int32, what="rowidx")
+            v = _dev_vec(f.val, torch.float64, ri.numel(), what="val")
             self._keep += [cp, ri, v]
             cps.append(cp.data_ptr())
             ris.append(ri.data_ptr())
@@ -289,6 +307,10 @@ class Batch:
             self._h = None
 
     def __del__(self):
+        # at interpreter exit the HIP runtime (and a profiler attached to
+        # it) may already be finalised: leave the memory to the process end
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -303,20 +325,22 @@ class Batch:
     def _tables(self, bs, xs):
         torch = _torch()
         nf = self.nf
-        if xs is None:
-            xs = [None if b is None else
-                  (torch.empty_like(b) if b.dim() == 1 else
-                   torch.empty((b.shape[1], b.shape[0]), dtype=b.dtype,
-                               device=b.device).t()) for b in bs]
+        if len(bs) != nf or (xs is not None and len(xs) != nf):
+            raise ValueError("one entry of bs (and xs) per factor")
+        bs = list(bs)
+        xs = [None] * nf if xs is None else list(xs)
+        self._call_keep = []  # converted right-hand sides live until return
         bp, xp, lb, lx, nr = [], [], [], [], []
         for f in range(nf):
             if bs[f] is None:
                 bp.append(None), xp.append(None)
                 lb.append(self.m[f]), lx.append(self.m[f]), nr.append(0)
                 continue
-            pb, ldb, k = _colmajor(torch, bs[f], self.m[f])
-            px, ldx, _ = _colmajor(torch, xs[f], self.m[f])
-            bp.append(pb.value), xp.append(px.value)
+            b, ldb, k = _rhs(torch, bs[f], self.m[f])
+            bs[f] = b
+            xs[f], ldx = _out(torch, xs[f], self.m[f], k, b.dim() == 2)
+            self._call_keep.append(b)
+            bp.append(b.data_ptr()), xp.append(xs[f].data_ptr())
             lb.append(ldb), lx.append(ldx), nr.append(k)
         P = ctypes.c_void_p * nf
         I = ctypes.c_int * nf

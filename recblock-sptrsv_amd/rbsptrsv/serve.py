@@ -137,6 +137,9 @@ class Server:
             for f in range(nf):
                 if busy[f] is not None and busy[f][0].query():
                     ev, rs, _, _ = busy[f]
+                    # the batch's answers are only handed out when the
+                    # plan's watchdog is clear (raises RbError otherwise)
+                    self.plans[f].check(self._stream(f))
                     t = start.elapsed_time(ev)
                     for r in rs:
                         r.done_ms = t

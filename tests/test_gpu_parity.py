@@ -622,3 +622,37 @@ def test_serving_results_same_alone_and_in_the_mix(torch, mode):
     for r in out2:
         assert residual(facs[r.factor], r.x.cpu().numpy(),
                         r.b.cpu().numpy()) < RES_TOL
+
+
+@pytest.mark.parametrize("schedule", [1, 2])
+def test_watchdog_trip_gives_all_nan_and_check_raises(torch, schedule):
+    """Force the failure path from the host: a plan whose bounded waits give
+    up after one poll.  Every kernel still exits normally; the solve must
+    then hand out x with every entry NaN (never a finite wrong answer),
+    rb_plan_check must report RB_ERR_INTERNAL and clear the flag, and a
+    healthy plan on the same factor is unaffected."""
+    from rbsptrsv import Plan
+    from rbsptrsv._lib import RbError
+    if schedule == 1:   # lane k of a wave waits on lane k-1: always waits
+        f = csc_from_deps(3000, {i: [i - 1] for i in range(1, 3000)})
+    else:               # cross-wave dependencies on every level
+        f = gen.banded(20000, seed=4)
+    print("watchdog trip, schedule", schedule, flush=True)
+    b = torch.as_tensor(gen.rhs(f.n, seed=2), device="cuda")
+    p = Plan.from_factor(f, lv=1, schedule=schedule, watchdog_spins=1)
+    try:
+        x = p.solve(b)
+        with pytest.raises(RbError) as e:
+            p.check()
+        assert e.value.status == 8
+        assert bool(torch.isnan(x).all())
+        p.check()   # the flag was cleared by the report
+    finally:
+        p.close()
+    q = Plan.from_factor(f, lv=1, schedule=schedule)
+    try:
+        y = q.solve(b)
+        q.check()
+        assert residual(f, y.cpu().numpy(), b.cpu().numpy()) < 1e-12
+    finally:
+        q.close()
