@@ -60,31 +60,79 @@ def make_workload(name):
     raise SystemExit("unknown workload %s" % name)
 
 
-def cpu_baseline(f, lv, budget_s):
-    """The oracle's block-ordered CPU solve (single thread) on the same
-    factor, repeated until ~budget_s of CPU work."""
+def host_cpu():
+    """(nproc, usable cores, CPU model) of this host."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return os.cpu_count(), usable, model
+
+
+def _median_time(fn, budget_s, min_reps=20, warm=3):
+    """Median seconds per call after `warm` warm-up calls, over at least
+    min_reps timed calls (fewer only if budget_s runs out first)."""
     import numpy as np
+    for _ in range(warm):
+        fn()
+    ts = []
+    t_start = time.perf_counter()
+    while len(ts) < min_reps or (time.perf_counter() - t_start < 1.0 and
+                                 len(ts) < 200):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+        if time.perf_counter() - t_start > budget_s and len(ts) >= 3:
+            break
+    return float(np.median(ts)), len(ts)
+
+
+def cpu_baseline(f, lv, budget_s):
+    """The reference's algorithm on the host cores: the oracle's level
+    reorder, partition and block order, triangular blocks level by level
+    over OpenMP threads (serially when their levels are narrow), square
+    blocks as row-parallel SpMV (oracle/rb_cpu.c); timed on all threads
+    OpenMP is given and on one.  Beside it the plain serial CSR
+    substitution of the original matrix.  Medians after 3 warm-ups."""
     import oracle
     from rbsptrsv import gen
+    B = algorithmic_bytes(f.n, f.nnz)
     tp = time.perf_counter()
-    re, part, bnz, an = oracle.full_pipeline(f.colptr, f.rowidx, f.val, f.n,
-                                             f.substitution, lv)
+    oracle.full_pipeline(f.colptr, f.rowidx, f.val, f.n, f.substitution, lv)
     pre_ms = (time.perf_counter() - tp) * 1e3
+    solver = oracle.CpuSolver(f, lv)
     b = gen.rhs(f.n, seed=1)
-    t0 = time.perf_counter()
-    reps = 0
-    while True:
-        oracle.solve_blocks(re, f.n, lv, f.substitution, part, b)
-        reps += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s or reps >= 2000:
-            break
-    per = el / reps
-    return dict(value=round(algorithmic_bytes(f.n, f.nnz) / per / 1e9, 4),
-                unit="GB/s", cores=1, kind="port",
-                sample="%d block-ordered single-thread solves of the same "
-                       "workload (%.1f ms each)" % (reps, per * 1e3),
-                ms_per_solve=round(per * 1e3, 3),
+    nthr = oracle.cpu_threads()
+    t_all, n_all = _median_time(lambda: solver.solve(b, nthr), budget_s / 3)
+    t_one, n_one = _median_time(lambda: solver.solve(b, 1), budget_s / 3)
+    solver.close()
+    rp, ci, v = f.csr()
+    t_csr, n_csr = _median_time(
+        lambda: oracle.sptrsv_serial(rp, ci, v, f.n, f.substitution, b),
+        budget_s / 3)
+    nproc, usable, model = host_cpu()
+    gbs = lambda t: round(B / t / 1e9, 4)
+    return dict(value=gbs(t_all), unit="GB/s", cores=nthr, kind="port",
+                sample="median of %d block-ordered solves of the same "
+                       "workload on %d OpenMP threads (%.2f ms each)"
+                       % (n_all, nthr, t_all * 1e3),
+                ms_per_solve=round(t_all * 1e3, 3),
+                one_core={"value": gbs(t_one), "ms_per_solve":
+                          round(t_one * 1e3, 3), "reps": n_one},
+                serial_csr={"value": gbs(t_csr), "ms_per_solve":
+                            round(t_csr * 1e3, 3), "reps": n_csr,
+                            "what": "textbook row-oriented substitution of "
+                                    "the original CSR, one thread"},
+                nproc=nproc, usable_cores=usable, cpu_model=model,
                 # the reference's preprocessing restated on the CPU (level
                 # analysis, reorder, partition, block extraction and
                 # selector statistics), one thread, timed once
@@ -155,6 +203,8 @@ def bench_batch(args):
         b1 = [b[:, 0].contiguous() for b in bs]
         msp, _ = timed(lambda: [p.solve(b) for p, b in zip(plans, b1)], 3)
         say("plain: %.3f ms per 64 solves" % msp)
+        for p in plans:
+            p.close()
         plain = {"solves_per_s": round(len(facs) / (msp * 1e-3), 1),
                  "ms_per_64_solves": round(msp, 3),
                  "how": "one plan per factor (auto schedule), one "
@@ -188,17 +238,7 @@ def serve_factors():
     """16 resident factors, n log-spaced in [1e4, 4e6], recipes cycling
     banded / Laplacian lower / power-law."""
     from rbsptrsv import gen
-    facs = []
-    for i in range(16):
-        n = int(round(1e4 * 400 ** (i / 15)))
-        if i % 3 == 0:
-            facs.append(gen.banded(n, seed=100 + i))
-        elif i % 3 == 1:
-            side = int(round(n ** 0.5))
-            facs.append(gen.laplacian(side))
-        else:
-            facs.append(gen.powerlaw(n, seed=100 + i))
-    return facs
+    return gen.serve_factors(16, 1e4, 4e6, seed=100)
 
 
 def bench_serve(args):
@@ -343,9 +383,20 @@ def main():
     for _ in range(args.steps):
         plan.solve(db, dx)
         kms.append(plan.kernel_ms())
+    launch_ms = sum(kms) / len(kms)
+    # the same launch with a cold Infinity Cache (MALL, 256 MB): before each
+    # solve a 1 GiB buffer is written on the same stream, which evicts the
+    # factor's data from the MALL and the L2s
+    flush = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+    cold = []
+    for _ in range(max(3, min(args.steps, 20))):
+        flush.fill_(1.0)
+        plan.solve(db, dx)
+        cold.append(plan.kernel_ms())
+    del flush
+    cold_ms = sum(cold) / len(cold)
     plan.set_timing(False)
     plan.check()
-    launch_ms = sum(kms) / len(kms)
 
     # correctness of what was timed: relative residual of the last solve
     import numpy as np
@@ -383,7 +434,15 @@ def main():
                      "kernel": "k_fused" if sched == "fused" else
                                "blocked block walk (graph of block kernels)",
                      "launch_ms": round(launch_ms, 5),
-                     "algorithmic_bytes_per_launch": B},
+                     "algorithmic_bytes_per_launch": B,
+                     "cache": "warm: repeated solves, the factor's data "
+                              "may sit in the 256 MB Infinity Cache",
+                     "cold": {"launch_ms": round(cold_ms, 5),
+                              "achieved": round(B / (cold_ms * 1e-3) / 1e9, 3),
+                              "frac": round(B / (cold_ms * 1e-3) / 1e9 /
+                                            HBM_PEAK_GBS, 5),
+                              "how": "1 GiB written on the solve stream "
+                                     "before each timed launch"}},
     }
     if K > 1:
         out["metric"] = BASELINE_METRIC
@@ -400,6 +459,8 @@ def main():
                          "phases_ms": {k: round(info.phase_ms[i], 1) for i, k in
                                        enumerate(("levels", "blocks",
                                                   "fused_schedule", "tuning"))}}
+    plan.close()    # explicitly, before the interpreter (and any profiler
+                    # attached to the HIP runtime) shuts down
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(f, info.lv, args.cpu_budget)
         out["preprocess"]["cpu_port_ms"] = out["cpu_baseline"]["preprocess_ms"]

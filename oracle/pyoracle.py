@@ -17,12 +17,19 @@ _I = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _D = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 
 
+_CPU_PATH = os.path.join(_HERE, "libcpubase.so")
+_cpu = None
+
+
 def build(force=False):
-    """Compile liboracle.so with the committed Makefile (gcc only)."""
-    src = os.path.join(_HERE, "rb_oracle.c")
-    if force or not os.path.exists(_LIB_PATH) or \
-            os.path.getmtime(_LIB_PATH) < os.path.getmtime(src):
-        subprocess.check_call(["make", "-s", "-B", "-C", _HERE, "liboracle.so"])
+    """Compile liboracle.so and libcpubase.so with the committed Makefile
+    (gcc only)."""
+    for lib_path, src in ((_LIB_PATH, "rb_oracle.c"), (_CPU_PATH, "rb_cpu.c")):
+        src = os.path.join(_HERE, src)
+        if force or not os.path.exists(lib_path) or \
+                os.path.getmtime(lib_path) < os.path.getmtime(src):
+            subprocess.check_call(["make", "-s", "-B", "-C", _HERE,
+                                   os.path.basename(lib_path)])
     return _LIB_PATH
 
 
@@ -58,6 +65,66 @@ def lib():
         L.orc_spmv.restype = None
         _lib = L
     return _lib
+
+
+def cpu_lib():
+    """The CPU baseline (rb_cpu.c, OpenMP)."""
+    global _cpu
+    if _cpu is None:
+        build()
+        L = ctypes.CDLL(_CPU_PATH)
+        c_int, vp = ctypes.c_int, ctypes.c_void_p
+        L.cpu_plan_create.argtypes = [_I, _I, _D, c_int, c_int, c_int] + \
+            [_I] * 5 + [_I, _I, _I]
+        L.cpu_plan_create.restype = vp
+        L.cpu_plan_solve.argtypes = [vp, _D, _D, c_int]
+        L.cpu_plan_solve.restype = None
+        L.cpu_plan_destroy.argtypes = [vp]
+        L.cpu_plan_destroy.restype = None
+        L.cpu_max_threads.argtypes = []
+        L.cpu_max_threads.restype = c_int
+        _cpu = L
+    return _cpu
+
+
+class CpuSolver:
+    """The reference's block-ordered solve on the host cores (rb_cpu.c):
+    built from the oracle's level-ordered matrix, partition and local
+    levels.  ``solve(b, threads)`` returns x in the original order."""
+
+    def __init__(self, f, lv):
+        s = f.substitution
+        re, part, bnz, an = full_pipeline(f.colptr, f.rowidx, f.val, f.n,
+                                          s, lv)
+        self.m = f.n
+        lvp = np.concatenate([np.concatenate([lp, np.zeros(
+            int(part["blk_m"][2 * t]) + 1 - len(lp), np.int32)])
+            for t, lp in enumerate(an["lvptr_local"])]).astype(np.int32)
+        lvi = np.concatenate(an["lvitem_local"]).astype(np.int32) \
+            if self.m else np.zeros(1, np.int32)
+        self._h = cpu_lib().cpu_plan_create(
+            _i32(re["colptr"]), _i32(re["rowidx"]), _f64(re["val"]), f.n, lv,
+            s, part["up"], part["down"], part["left"], part["right"],
+            _i32(an["nlv"]), lvp, lvi, _i32(re["levelItem"]))
+        if not self._h:
+            raise ValueError("zero or missing diagonal")
+
+    def solve(self, b, threads=0):
+        x = np.zeros(self.m, np.float64)
+        cpu_lib().cpu_plan_solve(self._h, _f64(b), x, int(threads))
+        return x
+
+    def close(self):
+        if self._h:
+            cpu_lib().cpu_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+def cpu_threads():
+    return cpu_lib().cpu_max_threads()
 
 
 def _i32(a):

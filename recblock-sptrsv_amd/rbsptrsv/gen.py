@@ -179,6 +179,22 @@ def batch_factors(count=64, seed=0, nmin=1_000, nmax=500_000):
     return facs, [int(r) for r in nrhs]
 
 
+def serve_factors(count=16, nmin=1e4, nmax=4e6, seed=100):
+    """Resident factors of the serving workload: n log-spaced in
+    [nmin, nmax], recipes cycling banded / Laplacian lower (the nearest
+    square grid) / power-law, seeds seed + i."""
+    facs = []
+    for i in range(count):
+        n = int(round(nmin * (nmax / nmin) ** (i / max(1, count - 1))))
+        if i % 3 == 0:
+            facs.append(banded(n, seed=seed + i))
+        elif i % 3 == 1:
+            facs.append(laplacian(int(round(n ** 0.5))))
+        else:
+            facs.append(powerlaw(n, seed=seed + i))
+    return facs
+
+
 def rhs(n, seed=0, k=1):
     """Seeded right-hand sides U(-1,1), column-major (n, k)."""
     rng = np.random.default_rng(seed + 7919)
