@@ -229,6 +229,22 @@ rb_status rb_solve_f32ir(rb_plan *plan, const double *d_b, int ldb,
                          int max_iter, void *stream, int *iters,
                          double *relres);
 
+/* rb_solve without device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate -- code under stream or graph
+ * capture, serving loops and harnesses that forbid hipMalloc.  Same
+ * arguments, results and stream ordering as rb_solve (results are bitwise
+ * equal), but the per-solve scratch (permuted b and x and tickets of the
+ * blocked schedule, slot-order x and b/diag buffers of the fused schedule)
+ * lives in one fixed device array of rb_noalloc_limit() bytes (64 MiB) per
+ * device, so a plan whose single column needs more is rejected with
+ * RB_ERR_INVALID_ARG; columns go through in groups that fit.  The call
+ * allocates nothing and never synchronises.  Every call on a device shares
+ * that array: two calls that overlap on one device (other streams, other
+ * threads) race and must be ordered by the caller. */
+rb_status rb_solve_noalloc(rb_plan *plan, const double *d_b, int ldb,
+                           double *d_x, int ldx, int nrhs, void *stream);
+size_t rb_noalloc_limit(void);
+
 /* ---------------------------------------------------------------------
  * Batches: many independent factors of uneven size, preprocessed as one
  * block-diagonal plan, so that one persistent launch (fused schedule) or

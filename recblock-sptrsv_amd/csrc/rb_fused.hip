@@ -189,6 +189,7 @@ struct FusedArgs {
     const int *pub;      // [npub] published slots, grouped by wave
     double *xq;          // [nr][nsteps * 64] x in slot order (this solve)
     double *xq_next;     // the other buffers, re-armed for the next solve
+                         // (null: nothing to re-arm)
     const double *brec;  // [nr][nsteps * 64] b/diag in slot order
     long nslots;         // nsteps * 64: stride between right-hand sides
     int W;               // consumer waves per right-hand side
@@ -444,10 +445,14 @@ __device__ __forceinline__ void resolve(const FusedArgs &a, int w, int r,
                                         int *gtmp, double *ctmp, Shared &sh) {
     const double *xq = a.xq + r * a.nslots;
     // re-arm this wave's published slots of the next solve's buffer; no
-    // wave of this launch reads that buffer
-    unsigned long long *nxt = (unsigned long long *)(a.xq_next + r * a.nslots);
-    for (int i = a.pub0[w] + lane; i < a.pub0[w + 1]; i += 64)
-        nxt[a.pub[i]] = kPending;
+    // wave of this launch reads that buffer (none when the caller fills
+    // every slot with the sentinel before each launch)
+    if (a.xq_next) {
+        unsigned long long *nxt =
+            (unsigned long long *)(a.xq_next + r * a.nslots);
+        for (int i = a.pub0[w] + lane; i < a.pub0[w + 1]; i += 64)
+            nxt[a.pub[i]] = kPending;
+    }
 
     const int e0 = a.gent0[w], e1 = a.gent0[w + 1];
     double carry = 0.0;
@@ -1412,6 +1417,44 @@ static int rhs_per_launch(FusedPlan *F) {
     return std::max(1, std::min(64, slots / std::max(1, F->W)));
 }
 
+// One launch group: nr right-hand sides (b, x with leading dimensions ldb,
+// ldx) through prologue, k_fused and epilogue.  cur: nr slot-order x
+// buffers whose published slots hold the sentinel; nxt: the buffers of the
+// next solve, whose published slots the launch re-arms (may be null); brec:
+// nr slot-order b/diag buffers.  Buffers of one right-hand side are nslots
+// apart.
+static rb_status launch_group(rb_plan *p, const double *b, int ldb, double *x,
+                              int ldx, int nr, double *cur, double *nxt,
+                              double *brec, hipStream_t s) {
+    FusedPlan *F = p->fused;
+    const size_t nslots = (size_t)F->nsteps * 64;
+    const dim3 tiles((unsigned)((nslots + kTile - 1) / kTile), (unsigned)nr);
+    k_fused_prologue<<<tiles, kTileThreads, 0, s>>>(
+        F->d_trow, F->d_tperm, F->d_tinvd, (long)nslots, b, ldb, brec);
+    RB_HIP(hipGetLastError());
+    FusedArgs args{F->d_step0, F->d_gent0, F->d_pub0, F->d_rec,
+                   F->d_gsrc, F->d_gval, F->d_gmeta, F->d_pub, cur, nxt,
+                   brec, (long)nslots, F->W, p->d_status,
+                   p->spin_limit, F->d_trace};
+    void *kargs[] = {(void *)&args};
+    mark_start(p, s);
+    const bool ordinary = F->W * nr <= small_grid(F->ncu);
+    const void *kern = pick_kernel(F, nr > 1);
+    hipError_t e =
+        ordinary
+            ? hipLaunchKernel(kern, dim3(F->W * nr), dim3(kWG), kargs, 0, s)
+            : hipLaunchCooperativeKernel(kern, dim3(F->W * nr), dim3(kWG),
+                                         kargs, 0, s);
+    if (e != hipSuccess)
+        return status_of_hip(e, ordinary ? "hipLaunchKernel(k_fused)"
+                                         : "hipLaunchCooperativeKernel(k_fused)");
+    mark_end(p, s);
+    k_fused_epilogue<<<tiles, kTileThreads, 0, s>>>(
+        F->d_trow, F->d_tperm, (long)nslots, cur, x, ldx, p->d_status);
+    RB_HIP(hipGetLastError());
+    return RB_OK;
+}
+
 rb_status solve_fused(rb_plan *p, const double *d_b, int ldb, double *d_x,
                       int ldx, int nrhs, hipStream_t s) {
     FusedPlan *F = p->fused;
@@ -1436,37 +1479,52 @@ rb_status solve_fused(rb_plan *p, const double *d_b, int ldb, double *d_x,
                     cur + r * nslots, (long)nslots);
                 RB_HIP(hipGetLastError());
             }
-        const double *b = d_b + (size_t)k0 * ldb;
-        const dim3 tiles((unsigned)((nslots + kTile - 1) / kTile), (unsigned)nr);
-        k_fused_prologue<<<tiles, kTileThreads, 0, s>>>(
-            F->d_trow, F->d_tperm, F->d_tinvd, (long)nslots, b, ldb, F->d_brec);
-        RB_HIP(hipGetLastError());
-        FusedArgs args{F->d_step0, F->d_gent0, F->d_pub0, F->d_rec,
-                       F->d_gsrc, F->d_gval, F->d_gmeta, F->d_pub, cur, nxt,
-                       F->d_brec, (long)nslots, F->W, p->d_status,
-                       p->spin_limit, F->d_trace};
-        void *kargs[] = {(void *)&args};
-        mark_start(p, s);
-        const bool ordinary = F->W * nr <= small_grid(F->ncu);
-        const void *kern = pick_kernel(F, nr > 1);
-        hipError_t e =
-            ordinary
-                ? hipLaunchKernel(kern, dim3(F->W * nr), dim3(kWG), kargs, 0, s)
-                : hipLaunchCooperativeKernel(kern, dim3(F->W * nr), dim3(kWG),
-                                             kargs, 0, s);
-        if (e != hipSuccess)
-            return status_of_hip(e, ordinary ? "hipLaunchKernel(k_fused)"
-                                             : "hipLaunchCooperativeKernel(k_fused)");
-        mark_end(p, s);
-        k_fused_epilogue<<<tiles, kTileThreads, 0, s>>>(
-            F->d_trow, F->d_tperm, (long)nslots, cur, d_x + (size_t)k0 * ldx, ldx,
-            p->d_status);
-        RB_HIP(hipGetLastError());
+        RB_TRY(launch_group(p, d_b + (size_t)k0 * ldb, ldb,
+                            d_x + (size_t)k0 * ldx, ldx, nr, cur, nxt,
+                            F->d_brec, s));
         for (int r = 0; r < nr; r++) {
             F->armed[cb][r] = 0;
             F->armed[nb][r] = 1;
         }
         F->epoch++;
+    }
+    return RB_OK;
+}
+
+size_t fused_slots(const rb_plan *p) {
+    return p->fused ? (size_t)p->fused->nsteps * 64 : 0;
+}
+
+rb_status solve_fused_ws(rb_plan *p, const double *d_b, int ldb, double *d_x,
+                         int ldx, int nrhs, hipStream_t s, double *ws,
+                         size_t ws_bytes) {
+    FusedPlan *F = p->fused;
+    if (!F) {
+        set_error(RB_ERR_INTERNAL, "fused schedule not built");
+        return RB_ERR_INTERNAL;
+    }
+    if (p->m == 0) return RB_OK;
+    const size_t nslots = (size_t)F->nsteps * 64;
+    const size_t per_rhs = 2 * nslots * sizeof(double);
+    device_shape(F);
+    const int fit = (int)std::min<size_t>(64, ws_bytes / per_rhs);
+    if (fit < 1) {
+        set_error(RB_ERR_INVALID_ARG, "workspace too small for one column");
+        return RB_ERR_INVALID_ARG;
+    }
+    const int cap = std::min(fit, nrhs > 1 ? rhs_per_launch(F) : 1);
+    for (int k0 = 0; k0 < nrhs; k0 += cap) {
+        const int nr = std::min(cap, nrhs - k0);
+        double *cur = ws;
+        double *brec = ws + (size_t)nr * nslots;
+        // the workspace keeps whatever the last call left: every slot of
+        // every column goes back to the sentinel, and nothing is re-armed
+        const long n = (long)nr * (long)nslots;
+        k_fill_pending<<<(int)((n + 255) / 256), 256, 0, s>>>(cur, n);
+        RB_HIP(hipGetLastError());
+        RB_TRY(launch_group(p, d_b + (size_t)k0 * ldb, ldb,
+                            d_x + (size_t)k0 * ldx, ldx, nr, cur, nullptr,
+                            brec, s));
     }
     return RB_OK;
 }

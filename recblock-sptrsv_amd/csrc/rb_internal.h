@@ -187,6 +187,18 @@ rb_status build_fused(rb_plan *p, const int *d_prptr, const int *d_ccol,
 rb_status solve_fused(rb_plan *p, const double *d_b, int ldb, double *d_x,
                       int ldx, int nrhs, hipStream_t s);
 void fused_destroy(rb_plan *p);
+// the same solves on caller-provided per-solve buffers (ws, ws_bytes)
+// instead of the plan's: the blocked walk needs blocked_ws_bytes(p) (one
+// column at a time, eager launches), the fused schedule 16 bytes per slot
+// (fused_slots(p)) and column of a launch; neither allocates
+size_t blocked_ws_bytes(const rb_plan *p);
+size_t fused_slots(const rb_plan *p);
+rb_status solve_blocked_ws(rb_plan *p, const double *d_b, int ldb,
+                           double *d_x, int ldx, int nrhs, hipStream_t s,
+                           void *ws, size_t ws_bytes);
+rb_status solve_fused_ws(rb_plan *p, const double *d_b, int ldb, double *d_x,
+                         int ldx, int nrhs, hipStream_t s, double *ws,
+                         size_t ws_bytes);
 // events around a solve's main launch when timing is on
 inline void mark_start(rb_plan *p, hipStream_t s) {
     if (p->timing) (void)hipEventRecord(p->ev0, s);

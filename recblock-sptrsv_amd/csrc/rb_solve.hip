@@ -394,7 +394,7 @@ __global__ void k_residual(const int *__restrict__ rptr,
 // buffers only, so it can be captured once into a graph and replayed.
 template <class T>
 static void launch_blocks(rb_plan *p, const T *val, const T *invd, T *bperm,
-                          T *xperm, hipStream_t s) {
+                          T *xperm, int *tickets, hipStream_t s) {
     for (int i = 0; i < p->nblocks; i++) {
         const Block &B = p->blocks[i];
         if (B.blk_m == 0) continue;
@@ -423,7 +423,7 @@ static void launch_blocks(rb_plan *p, const T *val, const T *invd, T *bperm,
             default:  // 1 and 3
                 k_tri_syncfree<T><<<nblk((long)B.blk_m, 64), 64, 0, s>>>(
                     p->d_rowptr, p->d_col, val, invd, bperm, B.slot_base,
-                    B.up, B.blk_m, p->subst, p->d_tickets + i / 2, xperm,
+                    B.up, B.blk_m, p->subst, tickets + i / 2, xperm,
                     p->d_status, p->spin_limit);
                 break;
             }
@@ -485,7 +485,7 @@ static bool capture_blocks(rb_plan *p, void **exec, const T *val,
     bool ok = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) == hipSuccess;
     if (ok) ok = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) == hipSuccess;
     if (ok) {
-        launch_blocks<T>(p, val, invd, bperm, xperm, cs);
+        launch_blocks<T>(p, val, invd, bperm, xperm, p->d_tickets, cs);
         ok = hipStreamEndCapture(cs, &g) == hipSuccess && g != nullptr;
     }
     if (ok) ok = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) == hipSuccess;
@@ -519,7 +519,7 @@ static rb_status walk(rb_plan *p, void **exec, const T *val, const T *invd,
     if (graph)
         RB_HIP(hipGraphLaunch((hipGraphExec_t)*exec, s));
     else
-        launch_blocks<T>(p, val, invd, bperm, xperm, s);
+        launch_blocks<T>(p, val, invd, bperm, xperm, p->d_tickets, s);
     mark_end(p, s);
     k_epilogue<T><<<nblk(m), kT, 0, s>>>(xperm, p->d_levelItem, m, x,
                                          p->d_status);
@@ -535,6 +535,41 @@ rb_status solve_blocked(rb_plan *p, const double *d_b, int ldb, double *d_x,
                             p->d_bperm, p->d_xperm, d_b + (size_t)k * ldb,
                             d_x + (size_t)k * ldx, s));
     return RB_OK;
+}
+
+// The block walk on caller-provided per-solve buffers, launched kernel by
+// kernel (a captured graph holds the plan's own buffers): ws holds the
+// permuted b and x (m values each) and the triangular blocks' tickets.
+rb_status solve_blocked_ws(rb_plan *p, const double *d_b, int ldb,
+                           double *d_x, int ldx, int nrhs, hipStream_t s,
+                           void *ws, size_t ws_bytes) {
+    if (p->m == 0) return RB_OK;
+    const int m = p->m;
+    if (blocked_ws_bytes(p) > ws_bytes) {
+        set_error(RB_ERR_INVALID_ARG, "workspace too small for the plan");
+        return RB_ERR_INVALID_ARG;
+    }
+    double *bperm = (double *)ws;
+    double *xperm = bperm + m;
+    int *tickets = (int *)(xperm + m);
+    for (int k = 0; k < nrhs; k++) {
+        const int pro = nblk(m > p->ntri ? m : p->ntri);
+        k_prologue<double><<<pro, kT, 0, s>>>(d_b + (size_t)k * ldb,
+                                              p->d_levelItem, m, bperm, xperm,
+                                              tickets, p->ntri);
+        mark_start(p, s);
+        launch_blocks<double>(p, p->d_val, p->d_invd, bperm, xperm, tickets,
+                              s);
+        mark_end(p, s);
+        k_epilogue<double><<<nblk(m), kT, 0, s>>>(
+            xperm, p->d_levelItem, m, d_x + (size_t)k * ldx, p->d_status);
+        RB_HIP(hipGetLastError());
+    }
+    return RB_OK;
+}
+
+size_t blocked_ws_bytes(const rb_plan *p) {
+    return 2 * sizeof(double) * (size_t)p->m + sizeof(int) * (size_t)p->ntri;
 }
 
 // FP32 solves refined in FP64 (rb_solve_f32ir): per right-hand side,

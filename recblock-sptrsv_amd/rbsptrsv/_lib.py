@@ -27,6 +27,7 @@ EXPORTS = [
     "rb_batch_create", "rb_batch_solve", "rb_batch_solve_f32ir",
     "rb_batch_plan", "rb_batch_destroy",
     "rb_mmio_info", "rb_mmio_data", "rb_mmio_triangle",
+    "rb_solve_noalloc", "rb_noalloc_limit",
 ]
 
 RB_OK = 0
@@ -94,6 +95,10 @@ def lib():
     L.rb_plan_create.restype = i
     L.rb_solve.argtypes = [vp, vp, i, vp, i, i, vp]
     L.rb_solve.restype = i
+    L.rb_solve_noalloc.argtypes = [vp, vp, i, vp, i, i, vp]
+    L.rb_solve_noalloc.restype = i
+    L.rb_noalloc_limit.argtypes = []
+    L.rb_noalloc_limit.restype = ctypes.c_size_t
     L.rb_plan_check.argtypes = [vp, vp]
     L.rb_plan_check.restype = i
     L.rb_plan_destroy.argtypes = [vp]
