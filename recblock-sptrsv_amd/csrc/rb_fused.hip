@@ -54,38 +54,33 @@ namespace rb {
 namespace {
 
 constexpr int kR = 32;          // LDS x ring depth (steps), a power of two
-constexpr int kU = 2;           // resolver: batches of 64 polled together
 constexpr int kZeroOff = kR * 64 * 8;  // byte offset of the ring's zero
 constexpr int kWG = 128;        // consumer wave + resolver wave
 
-// FIFO slot of a lane's group, without a select: with p the block's first
-// FIFO byte (below FF * 8 for a FIFO of FF groups) and g = int 1, the
-// slot's byte address is
-//   ((p + g) & sel) | (g & ~sel),   sel = int 0
-// (one bit-field insert).  A lane with a group has sel = FF * 8 - 1 and
-// g = 8 * its group's offset from the block's first group (groups of the
-// block's earlier steps and of lower lanes in this step); a lane without
-// one has sel = 0 and g = FF * 8, the zero slot behind the FIFO.  p + g
-// stays below 2 * FF * 8 <= 2^kBlockShift, so bits kBlockShift.. of sel
-// select zeros: they hold group counts -- in lane 0 of a block's first
-// step, the block's; in lane 63 of every step, the block's groups up to
-// and including the step.
-constexpr int kBlockShift = 15;
-constexpr int has_group_sel(int FF) { return FF * 8 - 1; }
-constexpr int no_group_off(int FF) { return FF * 8; }
+// The FIFO is indexed by step: slot (s mod kQ, lane) holds the partial sum
+// of lane's global entries at step s, or zero when it has none.  The
+// resolver writes a step's groups into their slots, the consumer reads its
+// lane's slot (no per-lane FIFO offset is stored anywhere) and writes zero
+// back for the step kQ later.  The resolver may fill step s once the
+// consumer has consumed step s - kQ; kQ is at least two blocks of steps.
+constexpr int kQ = 64;
+// a block-checked consumer reads a step's FIFO value this many steps early
+constexpr int kFP = 3;
+// per global entry: (step << kStepShift) | (lane << 1) | last-of-group
+constexpr int kStepShift = 7;
 
 // Per-lane record of one step, as 32-bit words; every double starts at an
 // even word so it is an aligned register pair of the load that brings it:
-//   [0] sel (+ group counts)  [1] FIFO offset
-//   NDPP 3: [2..3] coefficient of lane l (previous step), [4..5] of lane
-//           l-1, [6..7] of lane l+1; NDPP 1 (2): [2..3] lane l, [4..5]
+//   NDPP 3: [0..1] coefficient of lane l (previous step), [2..3] of lane
+//           l-1, [4..5] of lane l+1; NDPP 1 (2): [0..1] lane l, [2..3]
 //           lane l-1 (l+1)
 //   NRING ring sources (byte offsets into the ring), padded to even, then
 //   NRING coefficients
-// padded to whole 16-byte words; word k of step s for lane l lives at
-// rec[(s * K + k) * 64 + l].  Every loaded word is read (a last word with
-// two ints is loaded as 8 bytes): a loaded word the step does not use would
-// free its registers while the load is in flight.
+// padded to whole 16-byte words (none at all for NDPP 0 without a ring);
+// word k of step s for lane l lives at rec[(s * K + k) * 64 + l].  Every
+// loaded word is read (a last word with two ints is loaded as 8 bytes): a
+// loaded word the step does not use would free its registers while the
+// load is in flight.
 // Every coefficient is stored as -a_ij / a_ii, and the global entries'
 // values as a_ij / a_ii, so a row is x_i = b_i/a_ii + sum(coef * x_j) - fifo:
 // fused multiply-adds only, without a subtraction or a product by 1/diag
@@ -94,22 +89,22 @@ constexpr int no_group_off(int FF) { return FF * 8; }
 // prologue.
 
 // record prefetch distance D in steps for K words per lane; the records
-// of D + 1 steps are held in registers (4K each), and a wave's step count
-// is padded to a multiple of D + 1.  Loads and stores in flight stay below
-// the 63 the vector memory counter can track (D * (K + 2) < 63), and for
-// K > 2 the register ring stays near 140 registers so that three waves fit
-// a SIMD (6 workgroups per compute unit for several right-hand sides).
+// of D + 1 steps are held in registers (4K + 2 each), and a wave's step
+// count is padded to a multiple of D + 1.  A step issues K + 2 vector
+// memory operations (its record words, b/diag, the store of x); the ones
+// in flight stay within the 63 the vector memory counter tracks
+// (D * (K + 2) <= 60), and for K > 2 the register ring stays near 140
+// registers so that three waves fit a SIMD (several right-hand sides).
+// At these depths a step is not bound by memory latency but by the data
+// its instructions move through the SIMD: on the 1500^2 Laplacian (K = 1)
+// D = 15 ran 43 ns per step, D = 20 50 ns and D = 11 51 ns, and the block
+// of D + 1 steps is also the grain at which one wave waits for another.
 constexpr int prefetch_steps(int K) {
-    return K <= 2 ? 15 : K <= 3 ? 10 : K <= 4 ? 7 : K <= 5 ? 6 : K <= 7 ? 4
-         : K <= 8 ? 3 : 2;
+    return K == 0 ? 20 : K <= 2 ? 15 : K == 3 ? 10 : K == 4 ? 7
+         : K == 5 ? 6 : K <= 7 ? 4 : K == 8 ? 3 : 2;
 }
 constexpr int block_steps(int K) { return prefetch_steps(K) + 1; }
-// FIFO capacity in groups (a power of two): a block's groups (at most 64
-// per step) plus one resolver round must fit
-constexpr int fifo_groups(int K) {
-    return block_steps(K) * 64 + 64 * kU <= 1024 ? 1024 : 2048;
-}
-static_assert(2 * 2048 * 8 <= (1 << kBlockShift), "group count bits");
+static_assert(2 * block_steps(0) <= kQ, "FIFO too small for two blocks");
 
 // DPP formats: 0 none; 3 lanes l, l-1, l+1 of the previous step; 1 lanes
 // l, l-1 only; 2 lanes l, l+1 only (a grid's forward / backward solve
@@ -118,12 +113,12 @@ constexpr int dpp_ints(int ndpp) { return ndpp == 3 ? 6 : ndpp ? 4 : 0; }
 
 // record words per lane for a format
 constexpr int words_of(int ndpp, int nring) {
-    return (2 + dpp_ints(ndpp) + ((nring + 1) & ~1) + 2 * nring + 3) / 4;
+    return (dpp_ints(ndpp) + ((nring + 1) & ~1) + 2 * nring + 3) / 4;
 }
 
 template <int NDPP, int NRING>
 struct Fmt {
-    static constexpr int kSrc = 2 + dpp_ints(NDPP);
+    static constexpr int kSrc = dpp_ints(NDPP);
     static constexpr int kVal = kSrc + ((NRING + 1) & ~1);
     static constexpr int kInts = kVal + 2 * NRING;
     static constexpr int K = words_of(NDPP, NRING);
@@ -134,7 +129,7 @@ struct Fmt {
     static_assert(K == (kInts + 3) / 4, "record width");
     static constexpr int D = prefetch_steps(K);
     static constexpr int NB = D + 1;  // record buffers, steps per block
-    static constexpr int FF = fifo_groups(K);
+    static_assert(2 * NB <= kQ, "FIFO too small for two blocks");
 };
 
 __device__ inline int lds_load(const int *p) {
@@ -218,8 +213,8 @@ __device__ inline double dpp_from_upper(double v) {
 }
 
 struct Shared {
-    int ready;    // groups resolved (resolver -> consumer)
-    int cons;     // groups consumed (consumer -> resolver)
+    int ready;    // steps whose groups are all in the FIFO (resolver)
+    int cons;     // steps consumed (consumer)
     int abort;
 };
 
@@ -234,19 +229,14 @@ struct Shared {
 // the loop latch sees one register assignment of the ring.
 // BLK: a step has no branch (a branch costs a wave about as much as the
 // whole dependency chain of a step, taken or not): the FIFO is checked
-// once per block, for every group the block reads.  That is only safe when
-// no block waits, directly or through other blocks, for a row of a later
+// once per block, for every step of the block.  That is only safe when no
+// block waits, directly or through other blocks, for a row of a later
 // step of its own block -- the host checks the block wait graph for cycles
 // and otherwise builds with BLK off, which checks the FIFO at every step.
-// Each lane's record holds its group's FIFO offset from the block's first
-// group; consumption is reported once per block.
 template <int NDPP, int NRING, bool BLK>
 struct Consumer {
     using F = Fmt<NDPP, NRING>;
-    static constexpr int K = F::K, D = F::D, NB = F::NB, FF = F::FF;
-    // a block's groups plus one resolver round fit the FIFO, so the
-    // resolver never waits for space the consumer is waiting for
-    static_assert(NB * 64 + 64 * kU <= FF, "FIFO too small for a block");
+    static constexpr int K = F::K, D = F::D, NB = F::NB;
 
     const FusedArgs &a;
     const int lane, ns, last;
@@ -254,20 +244,19 @@ struct Consumer {
     // this wave's records, slot-order x and b/diag
     const __amdgpu_buffer_rsrc_t rec, xq, br;
     double *xring;
-    const double *fifo;
+    double *fifo;
     Shared &sh;
-    int4 rb[NB][K];           // ring of step records
-    double bb[NB];            // ring of b/diag
-    int fifo_pos = 0, fifo_avail = 0, blk_end = 0;
-    unsigned pos8 = 0;     // FIFO byte of the block's first group
-    double fv_next = 0.0;  // FIFO value of the next step (BLK)
+    int4 rb[NB][K > 0 ? K : 1];  // ring of step records
+    double bb[NB];               // ring of b/diag
+    int fifo_avail = 0;          // steps ready (scalar)
+    double fq[NB];               // FIFO values of the block's steps (BLK)
     double xprev = 0.0;
     bool ok = true;
     unsigned long long t_wait = 0;
 
     __device__ __forceinline__ Consumer(const FusedArgs &a_, int w, int r,
                                         int lane_, double *xring_,
-                                        const double *fifo_, Shared &sh_)
+                                        double *fifo_, Shared &sh_)
         : a(a_), lane(lane_), ns(a_.step0[w + 1] - a_.step0[w]),
           last(a_.step0[w + 1] - a_.step0[w] - 1),
           lane16(lane_ * 16), lane8(lane_ * 8),
@@ -277,11 +266,12 @@ struct Consumer {
                       (long)ns * 512)),
           xring(xring_), fifo(fifo_), sh(sh_) {}
 
-    // the FIFO slot of a lane's group, or the zero slot (record ints 0, 1)
-    __device__ __forceinline__ double fifo_at(int sel, int g) const {
-        const unsigned a = pos8 + (unsigned)g;
-        const unsigned o = (a & (unsigned)sel) | ((unsigned)g & ~(unsigned)sel);
-        return *(const double *)((const char *)fifo + o);
+    // the lane's FIFO slot of step s; read, then cleared for step s + kQ
+    __device__ __forceinline__ double take_fifo(int s) {
+        double *p = fifo + (s & (kQ - 1)) * 64 + lane;
+        const double v = *p;
+        *p = 0.0;
+        return v;
     }
 
     __device__ __forceinline__ void load_rec(int B, int t) {
@@ -303,19 +293,25 @@ struct Consumer {
                 double, __builtin_amdgcn_raw_buffer_load_b64(br, lane8, tc * 512, 0));
     }
 
-    // records of steps 0 .. D-1, oldest first
+    // records of steps 0 .. D-1, oldest first.  Each is followed by a store
+    // through an empty buffer resource (no record is in range, so the
+    // memory unit drops it), so that the loop is entered with the same mix
+    // of loads and stores in flight as every later block has and the
+    // compiler's wait counts at the loop head are those of the steady state.
     template <int U>
     __device__ __forceinline__ void prime() {
         if constexpr (U < D) {
             load_rec(U, U);
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, 0.0),
+                                                  buf_rsrc(a.xq, 0), lane8, 0, 0);
             __builtin_amdgcn_sched_barrier(0);
             prime<U + 1>();
         }
     }
 
-    // wait until the resolver has published `need` groups; on a timeout or
-    // an abort elsewhere, mark the workgroup aborted (the remaining steps
-    // of the block then run through without waiting)
+    // wait until the resolver has every group of the steps below `need`;
+    // on a timeout or an abort elsewhere, mark the workgroup aborted (the
+    // remaining steps of the block then run through without waiting)
     __device__ __forceinline__ void wait_fifo(int need) {
         const unsigned long long tw = a.trace ? now() : 0;
         unsigned spins = 0;
@@ -343,7 +339,7 @@ struct Consumer {
             // the record of step s + D goes into the buffer of step s - 1
             load_rec((U + D) % NB, s + D);
             __builtin_amdgcn_sched_barrier(0);
-            int f[4 * K];
+            int f[4 * (K > 0 ? K : 1)];
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 f[4 * k] = rb[U][k].x;
@@ -352,34 +348,29 @@ struct Consumer {
                 f[4 * k + 3] = rb[U][k].w;
             }
             double bv = bb[U];
-            if constexpr (U == 0) {
-                // the counters are wave-uniform: keep them scalar
-                fifo_pos = __builtin_amdgcn_readfirstlane(fifo_pos);
+            if constexpr (U == 0 && BLK) {
+                // every step of the block must be ready; then the FIFO
+                // values of its first kFP steps are read
                 fifo_avail = __builtin_amdgcn_readfirstlane(fifo_avail);
-                pos8 = (unsigned)(fifo_pos & (FF - 1)) * 8;
-                blk_end = fifo_pos +
-                          (__builtin_amdgcn_readfirstlane(f[0]) >> kBlockShift);
-                // every group of the block must be in the FIFO
-                if constexpr (BLK)
-                    if (__builtin_expect(fifo_avail < blk_end, 0)) wait_fifo(blk_end);
+                if (__builtin_expect(fifo_avail < sb + NB, 0)) wait_fifo(sb + NB);
+#pragma unroll
+                for (int j = 0; j < kFP && j < NB; j++) fq[j] = take_fifo(sb + j);
             }
             if constexpr (!BLK) {
-                // the groups of this step and the block's earlier steps
-                const int need = fifo_pos + (__builtin_amdgcn_readlane(f[0], 63) >>
-                                             kBlockShift);
-                if (__builtin_expect(fifo_avail < need, 0)) wait_fifo(need);
+                fifo_avail = __builtin_amdgcn_readfirstlane(fifo_avail);
+                if (__builtin_expect(fifo_avail <= s, 0)) wait_fifo(s + 1);
             }
-            // FIFO values are read without a branch (lanes without a group
-            // read the zero slot).  With BLK every group of the block is
-            // present once the block has started, so the next step's value
-            // is read one step early and the LDS latency leaves the chain.
+            // With BLK every step of the block is ready once the block has
+            // started, so a step's FIFO value is read kFP steps early and
+            // the LDS latency leaves the chain.
             double fv;
-            if constexpr (BLK && U > 0) fv = fv_next;
-            else fv = fifo_at(f[0], f[1]);
-            if constexpr (BLK && U + 1 < NB) {
-                fv_next = fifo_at(rb[U + 1][0].x, rb[U + 1][0].y);
+            if constexpr (BLK) {
+                fv = fq[U];
+                if constexpr (U + kFP < NB) fq[U + kFP] = take_fifo(s + kFP);
                 // issue the read here, not behind this step's chain
                 __builtin_amdgcn_sched_barrier(0);
+            } else {
+                fv = take_fifo(s);
             }
             // with BLK the FIFO value is at hand (read a step early) and
             // goes in first; otherwise it is subtracted last, so the wait
@@ -398,13 +389,13 @@ struct Consumer {
             if constexpr (NDPP == 3) {
                 const double xl = dpp_from_lower(xprev);
                 const double xr = dpp_from_upper(xprev);
-                acc = __builtin_fma(dbl(f[2], f[3]), xprev, acc);
-                acc = __builtin_fma(dbl(f[4], f[5]), xl, acc);
-                acc = __builtin_fma(dbl(f[6], f[7]), xr, acc);
+                acc = __builtin_fma(dbl(f[0], f[1]), xprev, acc);
+                acc = __builtin_fma(dbl(f[2], f[3]), xl, acc);
+                acc = __builtin_fma(dbl(f[4], f[5]), xr, acc);
             } else if constexpr (NDPP == 1 || NDPP == 2) {
                 const double xn = NDPP == 1 ? dpp_from_lower(xprev) : dpp_from_upper(xprev);
-                acc = __builtin_fma(dbl(f[2], f[3]), xprev, acc);
-                acc = __builtin_fma(dbl(f[4], f[5]), xn, acc);
+                acc = __builtin_fma(dbl(f[0], f[1]), xprev, acc);
+                acc = __builtin_fma(dbl(f[2], f[3]), xn, acc);
             }
             const double xv = BLK ? acc : acc - fv;
             xprev = xv;
@@ -415,11 +406,10 @@ struct Consumer {
             __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, xv), xq,
                                                   lane8, s * 512, kSC1);
             if constexpr (U == NB - 1) {
-                // the block's groups are consumed: their FIFO slots may be
-                // refilled
-                fifo_pos = blk_end;
+                // the block's FIFO slots are cleared: the resolver may
+                // refill them
                 wg_release();
-                if (lane == 0) lds_store(&sh.cons, fifo_pos);
+                if (lane == 0) lds_store(&sh.cons, sb + NB);
             }
             step<U + 1>(sb);
         }
@@ -439,153 +429,165 @@ struct Consumer {
     }
 };
 
-template <int FF>
+__device__ inline unsigned long long rotr64(unsigned long long x, int r) {
+    return r ? (x >> r) | (x << (64 - r)) : x;
+}
+
+// Resolver wave: polls the wave's global entries in consumption order
+// ((step, lane) order), sums each group (the global entries of one row) in
+// entry order, writes it to the row's FIFO slot, and publishes in `ready`
+// the first step that still lacks a group.
+// The entries in flight form a window of 64 consecutive ones starting at
+// the first unpublished entry `base`: lane l holds the entries e0 + l +
+// 64 k, so when a prefix of the window is published, exactly those lanes
+// move on to their next entry (its metadata is already in registers) and
+// poll it at once, while the others poll again.  Every round -- one trip to
+// memory -- publishes whatever became ready during the last one; there is
+// no pause at batch boundaries.  An entry whose step the FIFO cannot hold
+// yet (kQ steps beyond the consumer) waits like a pending one.  A group
+// longer than the window is summed across rounds (`carry`).
 __device__ __forceinline__ void resolve(const FusedArgs &a, int w, int r,
                                         int lane, double *fifo, double *rtmp,
                                         int *gtmp, double *ctmp, Shared &sh) {
     const double *xq = a.xq + r * a.nslots;
-    // re-arm this wave's published slots of the next solve's buffer; no
-    // wave of this launch reads that buffer (none when the caller fills
-    // every slot with the sentinel before each launch)
+    const int e0 = a.gent0[w], e1 = a.gent0[w + 1];
+    constexpr int kAll = 1 << 30;
+    // steps before the first group need nothing
+    if (lane == 0)
+        lds_store(&sh.ready, e0 < e1 ? a.gmeta[e0] >> kStepShift : kAll);
+    // re-arm this wave's published slots of the next solve's buffer (no
+    // wave of this launch reads that buffer; nothing to do when the caller
+    // fills every slot with the sentinel before each launch), eight slots
+    // per lane in flight: done one slot at a time this delayed the first
+    // polls by ~10 us on the 1500^2 Laplacian
     if (a.xq_next) {
         unsigned long long *nxt =
             (unsigned long long *)(a.xq_next + r * a.nslots);
-        for (int i = a.pub0[w] + lane; i < a.pub0[w + 1]; i += 64)
-            nxt[a.pub[i]] = kPending;
+        const int p0 = a.pub0[w], p1 = a.pub0[w + 1];
+        for (int i0 = p0; i0 < p1; i0 += 8 * 64) {
+            int sl[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const int i = i0 + j * 64 + lane;
+                sl[j] = i < p1 ? a.pub[i] : -1;
+            }
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+                if (sl[j] >= 0) nxt[sl[j]] = kPending;
+        }
     }
-
-    const int e0 = a.gent0[w], e1 = a.gent0[w + 1];
+    unsigned long long t_poll = 0, nb = 0, nround = 0;
+    const unsigned long long t0 = a.trace ? now() : 0;
+    int e = e0 + lane;  // this lane's entry
+    int src = 0, gm = 0, src2 = 0, gm2 = 0;
+    double gv = 0.0, gv2 = 0.0;  // metadata of e and of e + 64
+    if (e < e1) {
+        src = a.gsrc[e];
+        gv = a.gval[e];
+        gm = a.gmeta[e];
+    }
+    if (e + 64 < e1) {
+        src2 = a.gsrc[e + 64];
+        gv2 = a.gval[e + 64];
+        gm2 = a.gmeta[e + 64];
+    }
+    double xv = e < e1 ? ld_x(&xq[src]) : 0.0;
+    int base = e0;
     double carry = 0.0;
-    int carry_g = -1;
-    int done_groups = 0;
-    bool abort = false;
-    unsigned long long t_rspace = 0, t_poll = 0, nb = 0, nround = 0, tw = 0;
-    for (int kb = e0; kb < e1 && !abort; kb += 64 * kU) {
-        // metadata and first polls of kU batches of 64 entries at once, so
-        // kU * 64 loads are in flight per round trip
-        if (a.trace) tw = now();
-        int srcu[kU], gmu[kU];
-        double gvu[kU], xvu[kU];
-#pragma unroll
-        for (int u = 0; u < kU; u++) {
-            const int idx = kb + 64 * u + lane;
-            srcu[u] = 0;
-            gmu[u] = 0;
-            gvu[u] = 0.0;
-            if (idx < e1) {
-                srcu[u] = a.gsrc[idx];
-                gvu[u] = a.gval[idx];
-                gmu[u] = a.gmeta[idx];
-            }
+    int carry_g = -1;  // key (step << 6 | lane) of the group carried over
+    unsigned spins = 0;
+    while (base < e1) {
+        nround++;
+        const int lim = lds_load_u(&sh.cons) + kQ;
+        const bool live = e < e1;
+        const bool pend = live && pending(xv);
+        const int key = gm >> 1, stp = gm >> kStepShift;
+        const bool hold = live && (pend || stp >= lim);
+        const int rot = (base - e0) & 63;   // lane of entry `base`
+        const int pos = (lane - rot) & 63;  // this lane's place in the window
+        const int avail = min(64, e1 - base);
+        // window bits: bit i is entry base + i
+        const unsigned long long mh = rotr64(__ballot(hold), rot);
+        const int fp = min(avail, mh ? (int)__ffsll((long long)mh) - 1 : 64);
+        int hi;  // publish window entries [0, hi)
+        if (fp >= avail) {
+            hi = avail;
+        } else {
+            const unsigned long long me =
+                rotr64(__ballot(live && (gm & 1)), rot) & ((1ull << fp) - 1);
+            hi = me ? 64 - __clzll((long long)me) : 0;
         }
-        {
-            // FIFO space for every group these batches may complete
-            const int nt = min(64 * kU, e1 - kb);
-            const int gmax = a.gmeta[kb + nt - 1] >> 1;
-            unsigned spins = 0;
-#pragma nounroll
-            while (gmax - lds_load_u(&sh.cons) >= FF) {
-                if (++spins > a.spin || lds_load_u(&sh.abort)) {
-                    abort = true;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-            }
-            if (abort) break;
-        }
-        if (a.trace) {
-            t_rspace += now() - tw;
-            tw = now();
-        }
-#pragma unroll
-        for (int u = 0; u < kU; u++)
-            xvu[u] = kb + 64 * u + lane < e1 ? ld_x(&xq[srcu[u]]) : 0.0;
-#pragma unroll
-        for (int u = 0; u < kU; u++) {
-            const int k0 = kb + 64 * u;
-            if (k0 >= e1 || abort) break;
+        bool adv = false;
+        if (hi > 0) {
             nb++;
-            const int n = min(64, e1 - k0);
-            const bool act = lane < n;
-            const int src = srcu[u], gm = gmu[u];
-            const double gv = gvu[u];
-            double xv = xvu[u];
-            const int g = gm >> 1;
-            unsigned spins = 0;
-            bool pend = act && pending(xv);
-            rtmp[lane] = act && !pend ? gv * xv : 0.0;
-            gtmp[lane] = act ? gm : -2;
-            const int gprev = __shfl_up(g, 1, 64);
-            // Publish progressively: groups whose entries lie in the ready
-            // prefix of the batch go out at once, so a pending entry of a
-            // later step never holds back an earlier step (that could close
-            // a wait cycle between waves).
-            int done_lo = 0;  // lanes below are summed and published
-            while (true) {
-                const unsigned long long pm = __ballot(pend);
-                const int fp = pm ? (int)__ffsll((long long)pm) - 1 : n;
-                int hi;  // sum lanes [done_lo, hi)
-                if (fp >= n) {
-                    hi = n;
-                } else {
-                    const unsigned long long em = __ballot(
-                        act && lane >= done_lo && lane < fp && (gtmp[lane] & 1));
-                    hi = em ? 64 - __clzll((long long)em) : done_lo;
+            spins = 0;
+            rtmp[lane] = live && !pend ? gv * xv : 0.0;
+            gtmp[lane] = live ? gm : -2;
+            const int kprev = __shfl(key, (lane + 63) & 63, 64);
+            // group heads sum their entries in order; a group begun in an
+            // earlier round starts from the carried sum
+            if (pos < hi && (pos == 0 || kprev != key)) {
+                double sum = (pos == 0 && carry_g == key) ? carry : 0.0;
+                int q = pos, l = lane;
+                while (true) {
+                    sum += rtmp[l];
+                    if (q + 1 >= hi || (gtmp[(l + 1) & 63] >> 1) != key) break;
+                    q++;
+                    l = (l + 1) & 63;
                 }
-                if (hi > done_lo) {
-                    // group heads sum their entries in order; a group begun
-                    // in an earlier batch starts from the carried sum
-                    const bool head = act && lane >= done_lo && lane < hi &&
-                                      (lane == done_lo || gprev != g);
-                    if (head) {
-                        double sum = (lane == 0 && carry_g == g) ? carry : 0.0;
-                        int l = lane;
-                        while (true) {
-                            sum += rtmp[l];
-                            if (l + 1 >= hi || (gtmp[l + 1] >> 1) != g) break;
-                            l++;
-                        }
-                        if (gtmp[l] & 1) fifo[g & (FF - 1)] = sum;  // complete
-                        else *ctmp = sum;  // only the batch's last group
-                    }
-                    const int gl = __shfl(gm, hi - 1, 64);
-                    if (gl & 1) {
-                        done_groups = (gl >> 1) + 1;
-                    } else {  // hi == n: the last group continues
-                        carry = *ctmp;
-                        done_groups = gl >> 1;
-                    }
-                    carry_g = (gl & 1) ? -1 : (gl >> 1);
-                    wg_release();
-                    if (lane == 0) lds_store(&sh.ready, done_groups);
-                    done_lo = hi;
-                }
-                if (fp >= n) break;
-                nround++;
-                ++spins;
-                if (spins > a.spin || lds_load_u(&sh.abort) ||
-                    ((spins & 1023) == 0 && status_now(a.status))) {
-                    if (lane == 0) {
-                        atomicOr(a.status, 4);
-                        lds_store(&sh.abort, 1);
-                    }
-                    abort = true;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-                if (pend) {
-                    xv = ld_x(&xq[src]);
-                    pend = pending(xv);
-                    if (!pend) rtmp[lane] = gv * xv;
-                }
+                if (gtmp[l] & 1)  // complete: into the row's slot
+                    fifo[(stp & (kQ - 1)) * 64 + (key & 63)] = sum;
+                else
+                    *ctmp = sum;  // the window's last group continues
             }
+            const int gl = __shfl(gm, (rot + hi - 1) & 63, 64);
+            int rdy;
+            if (gl & 1) {
+                carry_g = -1;
+                rdy = hi < avail ? __shfl(stp, (rot + hi) & 63, 64)
+                    : base + hi >= e1 ? kAll
+                    : __shfl(gm2, rot, 64) >> kStepShift;  // entry base + 64
+            } else {
+                carry = *ctmp;
+                carry_g = gl >> 1;
+                rdy = gl >> kStepShift;
+            }
+            wg_release();
+            if (lane == 0) lds_store(&sh.ready, rdy);
+            // the published lanes move on to their next entry
+            adv = live && pos < hi;
+            if (adv) {
+                e += 64;
+                src = src2;
+                gv = gv2;
+                gm = gm2;
+            }
+            base += hi;
+        } else {
+            ++spins;
+            if (spins > a.spin || lds_load_u(&sh.abort) ||
+                ((spins & 1023) == 0 && status_now(a.status))) {
+                if (lane == 0) {
+                    atomicOr(a.status, 4);
+                    lds_store(&sh.abort, 1);
+                }
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
         }
-        if (a.trace) t_poll += now() - tw;
-        if (lds_load_u(&sh.abort)) abort = true;
+        // new entries and pending ones poll; then the next metadata
+        if ((adv && e < e1) || (!adv && pend)) xv = ld_x(&xq[src]);
+        if (adv && e + 64 < e1) {
+            src2 = a.gsrc[e + 64];
+            gv2 = a.gval[e + 64];
+            gm2 = a.gmeta[e + 64];
+        }
+        if (lds_load_u(&sh.abort)) break;
     }
+    if (a.trace) t_poll = now() - t0;
     if (a.trace && lane == 0 && r == 0) {
         unsigned long long *T = a.trace + (size_t)w * 16;
-        T[7] = t_rspace;
+        T[7] = 0;
         T[8] = t_poll;
         T[9] = nb;
         T[10] = nround;
@@ -594,24 +596,22 @@ __device__ __forceinline__ void resolve(const FusedArgs &a, int w, int r,
 
 // Waves per SIMD the register allocation aims at.  MULTI (launches of
 // several right-hand sides, where co-resident workgroups decide how many
-// columns one launch takes): 3, i.e. 6 workgroups per compute unit, for the
-// block-checked kernels unless their LDS (a 2048-group FIFO plus the ring)
-// allows only 2.  Single right-hand sides and per-step-checked kernels keep
-// their registers (a tighter allocation costs the one wave ~2 % per step).
-constexpr int occupancy_target(int ndpp, int nring, bool blk, bool multi) {
-    return multi && blk && !(nring > 0 && fifo_groups(words_of(ndpp, nring)) > 1024)
-               ? 3 : 1;
+// columns one launch takes): 2 for the block-checked kernels (their LDS
+// holds a compute unit to about four workgroups).  Single
+// right-hand sides and per-step-checked kernels keep their registers (a
+// tighter allocation costs the one wave ~2 % per step).
+constexpr int occupancy_target(bool blk, bool multi) {
+    return multi && blk ? 2 : 1;
 }
 
 // One workgroup per (consumer wave w, right-hand side r): blockIdx.x =
 // r * W + w.  Right-hand sides share the records and nothing else.
 template <int NDPP, int NRING, bool BLK, bool MULTI>
 __global__ void __launch_bounds__(kWG)
-    __attribute__((amdgpu_waves_per_eu(occupancy_target(NDPP, NRING, BLK, MULTI))))
+    __attribute__((amdgpu_waves_per_eu(occupancy_target(BLK, MULTI))))
     k_fused(FusedArgs a) {
     __shared__ double xring[NRING > 0 ? kR * 64 + 1 : 1];
-    constexpr int FF = Fmt<NDPP, NRING>::FF;
-    __shared__ double fifo[FF + 1];  // + the zero slot
+    __shared__ double fifo[kQ * 64];
     __shared__ double rtmp[64];
     __shared__ int gtmp[64];
     __shared__ double ctmp;
@@ -620,19 +620,19 @@ __global__ void __launch_bounds__(kWG)
     const int w = blockIdx.x % a.W, r = blockIdx.x / a.W;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
+    for (int i = threadIdx.x; i < kQ * 64; i += kWG) fifo[i] = 0.0;
     if (threadIdx.x == 0) {
         sh.ready = 0;
         sh.cons = 0;
         sh.abort = 0;
         if (NRING > 0) xring[kR * 64] = 0.0;
-        fifo[FF] = 0.0;
     }
     __syncthreads();
     if (wave == 0) {
         Consumer<NDPP, NRING, BLK> c(a, w, r, lane, xring, fifo, sh);
         c.run(w, r);
     } else if (wave == 1) {
-        resolve<FF>(a, w, r, lane, fifo, rtmp, gtmp, &ctmp, sh);
+        resolve(a, w, r, lane, fifo, rtmp, gtmp, &ctmp, sh);
     }
 }
 
@@ -1038,7 +1038,10 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
         for (int w = 0; w < W; w++)
             while (steps[w].size() % Bp) steps[w].push_back(empty);
     }
-    const int kSrc = 2 + dpp_ints(NDPP), kVal = kSrc + ((NRING + 1) & ~1);
+    const int kSrc = dpp_ints(NDPP), kVal = kSrc + ((NRING + 1) & ~1);
+    // a global entry names its step in 24 bits
+    for (int w = 0; w < W; w++)
+        if (steps[w].size() >= (1u << 24)) return RB_OK;
     // ---- emit records
     std::vector<int> step0(W + 1, 0), gent0(W + 1, 0), pub0(W + 1, 0);
     for (int w = 0; w < W; w++) step0[w + 1] = step0[w] + (int)steps[w].size();
@@ -1051,7 +1054,7 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
     std::vector<int4, NoInit<int4>> rec((size_t)NS * K * 64);
     std::vector<int, NoInit<int>> origv((size_t)NS * 64);
     std::vector<double, NoInit<double>> sinvd((size_t)NS * 64);
-    const int NB = block_steps(K), FF = fifo_groups(K);
+    const int NB = block_steps(K);
     // Per wave, in parallel on host threads (waves write disjoint record,
     // slot and 1/diag ranges; their global entries, sources and block
     // wait edges are kept per wave and joined in wave order below).
@@ -1063,22 +1066,16 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
     std::vector<WaveOut> wout(W);
     auto emit_wave = [&](int w) {
         WaveOut &o = wout[w];
-        std::vector<int> f(4 * K);
+        std::vector<int> f(4 * std::max(K, 1));
         auto put = [&](int at, double v) {
             long long bits;
             std::memcpy(&bits, &v, 8);
             f[at] = (int)(bits & 0xffffffff);
             f[at + 1] = (int)(bits >> 32);
         };
-        int gord = 0, gofs = 0;
-        size_t bfirst = 0;
         for (int si = 0; si < (int)steps[w].size(); si++) {
             const auto &st = steps[w][si];
             const size_t sbase = (size_t)(step0[w] + si) * K * 64;
-            if (si % NB == 0) {
-                bfirst = sbase;
-                gofs = 0;  // offsets restart at every block
-            }
             for (int l = 0; l < 64; l++) {
                 const int q = st[l];
                 std::fill(f.begin(), f.end(), 0);
@@ -1093,9 +1090,9 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
                         const int u = qof(cc[e]);
                         if (NDPP && is_dpp(q, u)) {
                             const int du = lanew[u] - l;  // -1, 0, +1
-                            // lane l at [2..3]; the one side of NDPP 1 / 2
-                            // at [4..5]; NDPP 3: l-1 at [4..5], l+1 at [6..7]
-                            put(du == 0 ? 2 : (du < 0 || NDPP == 2) ? 4 : 6,
+                            // lane l at [0..1]; the one side of NDPP 1 / 2
+                            // at [2..3]; NDPP 3: l-1 at [2..3], l+1 at [4..5]
+                            put(du == 0 ? 0 : (du < 0 || NDPP == 2) ? 2 : 4,
                                 -cv[e] * invd[r]);
                         } else if (is_loc(q, u) && nl < NRING) {
                             f[kSrc + nl] = ((stepw[u] % kR) * 64 + lanew[u]) * 8;
@@ -1107,29 +1104,20 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
                                                  slot_of(u) / 64 / NB);
                             o.gsrc.push_back(slot_of(u));
                             o.gval.push_back(cv[e] * invd[r]);
-                            o.gmeta.push_back(gord * 2);
+                            o.gmeta.push_back(si << kStepShift | l << 1);
                             hasg = true;
                         }
                     }
-                    if (hasg) {
-                        o.gmeta.back() |= 1;
-                        gord++;
-                    }
+                    if (hasg) o.gmeta.back() |= 1;  // last of the row's group
                     sinvd[(size_t)(step0[w] + si) * 64 + l] = invd[r];
                 } else {  // padding lane
                     origv[(size_t)(step0[w] + si) * 64 + l] = -1;
                     sinvd[(size_t)(step0[w] + si) * 64 + l] = 0.0;
                 }
-                f[0] = hasg ? has_group_sel(FF) : 0;
-                f[1] = hasg ? gofs * 8 : no_group_off(FF);
-                gofs += hasg;
-                if (l == 63) f[0] |= gofs << kBlockShift;
                 for (int k = 0; k < K; k++)
                     rec[sbase + (size_t)k * 64 + l] =
                         make_int4(f[4 * k], f[4 * k + 1], f[4 * k + 2], f[4 * k + 3]);
             }
-            // the block's group count goes to lane 0 of its first step
-            if (si % NB == NB - 1) rec[bfirst].x |= gofs << kBlockShift;
         }
     };
     {
