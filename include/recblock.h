@@ -73,6 +73,20 @@ typedef struct {
                           RB_ERR_INTERNAL) while every kernel exits
                           normally.  No counterpart in the reference, whose
                           waits are unbounded. */
+    int block_kernels;  /* blocked schedule, kernel per block: 0 (default) =
+                          time the candidates on each block at plan build
+                          and keep the fastest (triangular: the reference
+                          selector's kernel, the sync-free grid kernel, the
+                          one-workgroup kernel when its dependencies fit its
+                          LDS ring; square: 1..64 lanes per row); 1 = the
+                          reference selector's kernels (method) and the
+                          lanes-per-row rule; 2 / 3 = every triangular block
+                          with the sync-free / one-workgroup kernel (where
+                          it applies).  Every choice gives the same x, bit
+                          for bit.  The reference's selector
+                          (R/recblocking_solver_cuda.h:176-367,441-546) fixes
+                          the kernel by thresholds; its decisions are still
+                          reported by rb_plan_blocks. */
 } rb_opts;
 
 typedef struct rb_plan rb_plan;
@@ -183,6 +197,14 @@ rb_status rb_plan_blocks(const rb_plan *plan, int *blk_m, int *blk_n,
                          int *right, int *method, int *nlv, int *nnzr,
                          int *m_new, int *lenmax, int *longrow, int *longlen,
                          double *empty_ratio);
+
+/* Kernels the blocked schedule runs per block (host arrays of nblocks,
+ * any may be NULL): kern (triangular: 0 the reference selector's kernel,
+ * 1 sync-free grid, 2 one workgroup; square: 0), lanes (square: lanes per
+ * row; triangular: 0), tuned_ms (the chosen kernel's time when tuned, else
+ * 0).  No counterpart in the reference. */
+rb_status rb_plan_block_kernels(const rb_plan *plan, int *kern, int *lanes,
+                                double *tuned_ms);
 
 /* Global level analysis (host arrays): levelItem[m] is the permutation that
  * maps level-order position i to original row levelItem[i] (reversed for

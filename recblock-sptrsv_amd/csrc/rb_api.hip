@@ -205,7 +205,12 @@ rb_status rb_plan_create(const int *d_colptr, const int *d_rowidx,
         st = build_plan(p, d_colptr, d_rowidx, d_val, (hipStream_t)stream);
     if (st == RB_OK) {
         auto t0 = std::chrono::steady_clock::now();
-        st = choose_schedule(p, (hipStream_t)stream);
+        // kernels per block for the blocked walk (not timed when only the
+        // fused schedule will run)
+        st = tune_blocks(p, o.schedule == RB_SCHED_FUSED && p->fused ? 1
+                                                                     : o.block_kernels,
+                         (hipStream_t)stream);
+        if (st == RB_OK) st = choose_schedule(p, (hipStream_t)stream);
         p->phase_ms[3] = std::chrono::duration<double, std::milli>(
                              std::chrono::steady_clock::now() - t0)
                              .count();
@@ -322,6 +327,21 @@ rb_status rb_plan_destroy(rb_plan *p) {
     fused_destroy(p);
     p->arena.release();
     delete p;
+    return ok();
+}
+
+rb_status rb_plan_block_kernels(const rb_plan *p, int *kern, int *lanes,
+                                double *tuned_ms) {
+    if (!p) {
+        set_error(RB_ERR_INVALID_ARG, "null plan");
+        return RB_ERR_INVALID_ARG;
+    }
+    for (int i = 0; i < p->nblocks; i++) {
+        const Block &B = p->blocks[i];
+        if (kern) kern[i] = B.kind == 0 ? B.kern : 0;
+        if (lanes) lanes[i] = B.kind == 1 ? B.lanes : 0;
+        if (tuned_ms) tuned_ms[i] = B.tuned_ms;
+    }
     return ok();
 }
 

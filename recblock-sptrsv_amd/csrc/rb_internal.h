@@ -92,7 +92,23 @@ struct Block {
     int lvptr_base;
     std::vector<int> lvptr_h;   // method 2: local level boundaries
     std::vector<int> nnzlv_h;   // method 2: find_level_nnz
+    // execution choice of the blocked walk (the selector's `method` above
+    // is kept as the reference decides it): triangular blocks run kern
+    // (kTriRef: the kernel of `method`), square blocks `lanes` lanes per
+    // row; set by the rule of the reference selector and then by timing
+    // (tune_blocks)
+    int kern = 0;
+    int lanes = 1;
+    int maxdist = 0;        // triangular: largest in-block dependency distance
+    double tuned_ms = 0.0;  // time of the chosen kernel when tuned
 };
+
+// triangular block kernels of the blocked walk
+constexpr int kTriRef = 0;       // method 0 / 2 / (1, 3) as the reference picks
+constexpr int kTriSyncfree = 1;  // sync-free CSR pull over the whole grid
+constexpr int kTriGroup = 2;     // one workgroup, x ring in LDS
+constexpr int kRingRows = 8192;  // kTriGroup's ring (positions)
+constexpr int kGroupThreads = 1024;
 
 struct FusedPlan;
 
@@ -181,6 +197,9 @@ rb_status solve_blocked(rb_plan *p, const double *d_b, int ldb, double *d_x,
 rb_status solve_f32ir(rb_plan *p, const double *d_b, int ldb, double *d_x,
                       int ldx, int nrhs, double tol, int max_iter,
                       hipStream_t s, int *iters, double *relres);
+// per-block kernel choice of the blocked walk by timing each candidate on
+// the plan's own buffers (rb_solve.hip); mode: rb_opts.block_kernels
+rb_status tune_blocks(rb_plan *p, int mode, hipStream_t s);
 // fused schedule (rb_fused.hip)
 rb_status build_fused(rb_plan *p, const int *d_prptr, const int *d_ccol,
                       const double *d_cval, int Wmax, hipStream_t s);

@@ -24,6 +24,10 @@
 // The kernels are templates on the value type: the FP64 walk is the plain
 // solve; the FP32 walk (FP32 copies of the slot values) is the inner solver
 // of the FP32 + FP64 iterative refinement at the end of this file.
+#include <algorithm>
+#include <utility>
+#include <vector>
+
 #include "rb_internal.h"
 
 namespace rb {
@@ -200,6 +204,121 @@ k_tri_syncfree(const int *__restrict__ rowptr, const int *__restrict__ col,
             }
         }
     }
+}
+
+// One workgroup solves the whole triangular block: thread t takes the
+// rows at positions t, t + 1024, ... (solve order: the slot order of a
+// forward block, reversed for a backward one, which the global level order
+// makes a topological order) and keeps a ring of the last kRingRows
+// solved rows' x in LDS, each slot tagged with its row's position
+// (release store after the value; acquire load before reading it).  A
+// row polls only the ring -- a dependency is an LDS read, not a trip
+// through device memory as in the sync-free grid kernel, nor a launch per
+// level as in the level-set kernels -- so deep, narrow blocks (hundreds of
+// short levels) run faster.  Correct when every in-block dependency is at
+// most kRingRows - kGroupThreads positions back: all rows in flight lie
+// within 1024 positions of the oldest unsolved one, so a slot is reused
+// only after every row that reads it has finished (the host checks the
+// distance).  Lanes of one wave may depend on each other, so a wave loops
+// until all its lanes are done, each lane advancing as far as its
+// dependencies allow; a row's entries are read ahead in chunks of kChunk,
+// before the waits, and summed in order, as by the other kernels.
+// (A level-synchronous form -- one workgroup barrier per level, rows read
+// 1024 positions ahead -- took ~1.1 us per level on the banded factor
+// against this form's ~0.9 us: a wave waits for its newest loads as a
+// whole, so the reads ahead for some lanes held the others at every
+// level.)
+template <class T>
+__global__ void __launch_bounds__(kGroupThreads)
+k_tri_group(const int *__restrict__ rowptr, const int *__restrict__ col,
+            const T *__restrict__ val, const T *__restrict__ invd,
+            const T *__restrict__ bperm, int sb, int up, int bm, int subst,
+            T *xperm, int *status, unsigned spin_limit) {
+    constexpr int kChunk = 8;
+    __shared__ int tag[kRingRows];
+    __shared__ T xr[kRingRows];
+    for (int i = threadIdx.x; i < kRingRows; i += kGroupThreads) tag[i] = -1;
+    __syncthreads();
+    bool bad = false;
+    for (int p0 = threadIdx.x & ~63; p0 < bm; p0 += kGroupThreads) {
+        const int pos = p0 + (threadIdx.x & 63);
+        const bool valid = pos < bm;
+        const int r = subst == 0 ? pos : bm - 1 - pos;
+        int jc = 0, e = 0;  // chunk base, row end
+        T bv = 0, iv = 0;
+        if (valid) {
+            jc = rowptr[sb + r];
+            e = rowptr[sb + r + 1];
+            bv = bperm[up + r];
+            iv = invd[up + r];
+        }
+        int cc[kChunk];
+        T vv[kChunk];
+        auto load_chunk = [&]() {
+#pragma unroll
+            for (int q = 0; q < kChunk; q++) {
+                cc[q] = jc + q < e ? col[jc + q] : 0;
+                vv[q] = jc + q < e ? val[jc + q] : T(0);
+            }
+        };
+        load_chunk();
+        int done = 0;  // entries of the chunk already summed
+        T acc = 0;
+        bool pend = valid;
+        unsigned spins = 0;
+        while (__any(pend)) {
+            if (pend) {
+                bool stall = false;
+                while (true) {
+#pragma unroll
+                    for (int q = 0; q < kChunk; q++) {
+                        if (!stall && q >= done && jc + q < e) {
+                            const int c = cc[q];
+                            const int pp = subst == 0 ? c : bm - 1 - c;
+                            const int k = pp & (kRingRows - 1);
+                            if (__hip_atomic_load(&tag[k], __ATOMIC_ACQUIRE,
+                                                  __HIP_MEMORY_SCOPE_WORKGROUP) == pp) {
+                                acc += vv[q] * xr[k];
+                                done = q + 1;
+                            } else {
+                                stall = true;
+                            }
+                        }
+                    }
+                    if (stall || jc + kChunk >= e) break;
+                    jc += kChunk;  // the chunk is summed: the next one
+                    done = 0;
+                    load_chunk();
+                }
+                if (!stall) {
+                    const T x = (bv - acc) * iv;
+                    const int k = pos & (kRingRows - 1);
+                    xr[k] = x;
+                    __hip_atomic_store(&tag[k], pos, __ATOMIC_RELEASE,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                    xperm[up + r] = x;
+                    pend = false;
+                }
+            }
+            if (__any(pend)) {
+                if (++spins > spin_limit) {
+                    // a dependency that never arrives: flag it, release the
+                    // row (the epilogue turns x into NaN)
+                    if (pend) {
+                        bad = true;
+                        const int k = pos & (kRingRows - 1);
+                        xr[k] = __builtin_bit_cast(T, Bits<T>::kQNaN);
+                        __hip_atomic_store(&tag[k], pos, __ATOMIC_RELEASE,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                        pend = false;
+                    }
+                } else {
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+        }
+    }
+    if (bad) atomicOr(status, 1);
 }
 
 // square block SpMV, CSR, thread per row; b[up + r] -= A[r,:] x[left:]
@@ -388,87 +507,116 @@ __global__ void k_residual(const int *__restrict__ rptr,
         atomicMax(rmax, (unsigned long long)__double_as_longlong(a));
 }
 
+// largest in-block dependency distance |row - column| of every triangular
+// block (blockIdx.y = triangular block; sb / bm its slots and rows)
+__global__ void k_tri_maxdist(const int *__restrict__ rowptr,
+                              const int *__restrict__ col,
+                              const int *__restrict__ sbs,
+                              const int *__restrict__ bms, int *out) {
+    const int t = blockIdx.y, sb = sbs[t], bm = bms[t];
+    int mx = 0;
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < bm;
+         r += gridDim.x * blockDim.x)
+        for (int j = rowptr[sb + r]; j < rowptr[sb + r + 1]; j++)
+            mx = max(mx, abs(r - col[j]));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(&out[t], mx);
+}
+
+// before a timed run of a triangular block: its x slots back to the
+// sentinel and its ticket to zero
+__global__ void k_block_reset(double *x, int n, int *ticket) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) ((unsigned long long *)x)[i] = kPending;
+    if (i == 0) *ticket = 0;
+}
+
 }  // namespace
+
+// One block of the walk in value type T (the kernels of B.kern / B.lanes).
+template <class T>
+static void launch_block(rb_plan *p, int i, const T *val, const T *invd,
+                         T *bperm, T *xperm, int *tickets, hipStream_t s) {
+    const Block &B = p->blocks[i];
+    if (B.blk_m == 0) return;
+    if (B.kind == 0) {
+        if (B.method == 0) {
+            k_tri_diag<T><<<nblk(B.blk_m), kT, 0, s>>>(invd, bperm, B.up,
+                                                       B.blk_m, xperm);
+        } else if (B.kern == kTriGroup) {
+            k_tri_group<T><<<1, kGroupThreads, 0, s>>>(
+                p->d_rowptr, p->d_col, val, invd, bperm, B.slot_base, B.up,
+                B.blk_m, p->subst, xperm, p->d_status, p->spin_limit);
+        } else if (B.kern == kTriRef && B.method == 2) {
+            for (int L = 0; L < B.nlv; L++) {
+                const int a = B.lvptr_h[L], c = B.lvptr_h[L + 1];
+                const int nr = c - a;
+                if (nr <= 0) continue;
+                const int *rows = p->d_lvitem + B.lv_base + a;
+                if (L == 0 || B.nnzlv_h[L] / nr <= 15)
+                    k_tri_level_thread<T><<<nblk(nr), kT, 0, s>>>(
+                        p->d_rowptr, p->d_col, val, invd, bperm, rows, nr,
+                        B.slot_base, B.up, xperm);
+                else
+                    k_tri_level_wave<T><<<nblk((long)nr * 64), kT, 0, s>>>(
+                        p->d_rowptr, p->d_col, val, invd, bperm, rows, nr,
+                        B.slot_base, B.up, xperm);
+            }
+        } else {  // methods 1 and 3, or kTriSyncfree
+            k_tri_syncfree<T><<<nblk((long)B.blk_m, 64), 64, 0, s>>>(
+                p->d_rowptr, p->d_col, val, invd, bperm, B.slot_base,
+                B.up, B.blk_m, p->subst, tickets + i / 2, xperm,
+                p->d_status, p->spin_limit);
+        }
+        return;
+    }
+    // The reference's method picks CSR (0, 2) or DCSR (1, 3) rows; lanes
+    // per row are the block's own (the reference's thread / warp split is
+    // 1 or a whole wave).
+    const int *rows = p->d_dcsr_rows + B.dcsr_base;
+    const bool dcsr = B.method == 1 || B.method == 3;
+    const int nr = dcsr ? B.m_new : B.blk_m;
+    const int g = B.lanes;
+    if (B.method < 0 || nr == 0) return;  // empty block
+    if (g == 1) {
+        if (dcsr)
+            k_sq_dcsr_thread<T><<<nblk(nr), kT, 0, s>>>(
+                p->d_rowptr, p->d_col, val, xperm, rows, nr, B.slot_base,
+                B.up, B.left, bperm);
+        else
+            k_sq_csr_thread<T><<<nblk(nr), kT, 0, s>>>(
+                p->d_rowptr, p->d_col, val, xperm, B.slot_base, B.up, B.left,
+                nr, bperm);
+    } else if (g == 64) {
+        if (dcsr)
+            k_sq_dcsr_wave<T><<<nblk((long)nr * 64), kT, 0, s>>>(
+                p->d_rowptr, p->d_col, val, xperm, rows, nr, B.slot_base,
+                B.up, B.left, bperm);
+        else
+            k_sq_csr_wave<T><<<nblk((long)nr * 64), kT, 0, s>>>(
+                p->d_rowptr, p->d_col, val, xperm, B.slot_base, B.up, B.left,
+                nr, bperm);
+    } else if (dcsr) {
+        launch_sq<T, true>(g, nr, s, p->d_rowptr, p->d_col, val, xperm, rows,
+                           B.slot_base, B.up, B.left, bperm);
+    } else {
+        launch_sq<T, false>(g, nr, s, p->d_rowptr, p->d_col, val, xperm,
+                            rows, B.slot_base, B.up, B.left, bperm);
+    }
+    if (B.longrow > 0)
+        k_sq_long<T><<<B.longrow, kT, 0, s>>>(
+            p->d_rowptr, p->d_col, val, xperm, p->d_long_rows + B.long_base,
+            B.slot_base, B.up, B.left, bperm);
+}
 
 // The block walk of one solve in value type T: kernels on plan-owned
 // buffers only, so it can be captured once into a graph and replayed.
 template <class T>
 static void launch_blocks(rb_plan *p, const T *val, const T *invd, T *bperm,
                           T *xperm, int *tickets, hipStream_t s) {
-    for (int i = 0; i < p->nblocks; i++) {
-        const Block &B = p->blocks[i];
-        if (B.blk_m == 0) continue;
-        if (B.kind == 0) {
-            switch (B.method) {
-            case 0:
-                k_tri_diag<T><<<nblk(B.blk_m), kT, 0, s>>>(invd, bperm, B.up,
-                                                           B.blk_m, xperm);
-                break;
-            case 2:
-                for (int L = 0; L < B.nlv; L++) {
-                    const int a = B.lvptr_h[L], c = B.lvptr_h[L + 1];
-                    const int nr = c - a;
-                    if (nr <= 0) continue;
-                    const int *rows = p->d_lvitem + B.lv_base + a;
-                    if (L == 0 || B.nnzlv_h[L] / nr <= 15)
-                        k_tri_level_thread<T><<<nblk(nr), kT, 0, s>>>(
-                            p->d_rowptr, p->d_col, val, invd, bperm, rows, nr,
-                            B.slot_base, B.up, xperm);
-                    else
-                        k_tri_level_wave<T><<<nblk((long)nr * 64), kT, 0, s>>>(
-                            p->d_rowptr, p->d_col, val, invd, bperm, rows, nr,
-                            B.slot_base, B.up, xperm);
-                }
-                break;
-            default:  // 1 and 3
-                k_tri_syncfree<T><<<nblk((long)B.blk_m, 64), 64, 0, s>>>(
-                    p->d_rowptr, p->d_col, val, invd, bperm, B.slot_base,
-                    B.up, B.blk_m, p->subst, tickets + i / 2, xperm,
-                    p->d_status, p->spin_limit);
-                break;
-            }
-        } else {
-            // The reference's method picks CSR (0, 2) or DCSR (1, 3) rows;
-            // lanes per row follow the block's mean row length (the
-            // reference's thread / warp split is 1 or a whole wave).
-            const int *rows = p->d_dcsr_rows + B.dcsr_base;
-            const bool dcsr = B.method == 1 || B.method == 3;
-            const int nr = dcsr ? B.m_new : B.blk_m;
-            const int g = lanes_per_row(B.nnzr);
-            if (B.method < 0 || nr == 0) {
-                // empty block
-            } else if (g == 1) {
-                if (dcsr)
-                    k_sq_dcsr_thread<T><<<nblk(nr), kT, 0, s>>>(
-                        p->d_rowptr, p->d_col, val, xperm, rows, nr,
-                        B.slot_base, B.up, B.left, bperm);
-                else
-                    k_sq_csr_thread<T><<<nblk(nr), kT, 0, s>>>(
-                        p->d_rowptr, p->d_col, val, xperm, B.slot_base, B.up,
-                        B.left, nr, bperm);
-            } else if (g == 64) {
-                if (dcsr)
-                    k_sq_dcsr_wave<T><<<nblk((long)nr * 64), kT, 0, s>>>(
-                        p->d_rowptr, p->d_col, val, xperm, rows, nr,
-                        B.slot_base, B.up, B.left, bperm);
-                else
-                    k_sq_csr_wave<T><<<nblk((long)nr * 64), kT, 0, s>>>(
-                        p->d_rowptr, p->d_col, val, xperm, B.slot_base, B.up,
-                        B.left, nr, bperm);
-            } else if (dcsr) {
-                launch_sq<T, true>(g, nr, s, p->d_rowptr, p->d_col, val, xperm,
-                                   rows, B.slot_base, B.up, B.left, bperm);
-            } else {
-                launch_sq<T, false>(g, nr, s, p->d_rowptr, p->d_col, val, xperm,
-                                    rows, B.slot_base, B.up, B.left, bperm);
-            }
-            if (B.method >= 0 && B.longrow > 0)
-                k_sq_long<T><<<B.longrow, kT, 0, s>>>(
-                    p->d_rowptr, p->d_col, val, xperm,
-                    p->d_long_rows + B.long_base, B.slot_base, B.up, B.left,
-                    bperm);
-        }
-    }
+    for (int i = 0; i < p->nblocks; i++)
+        launch_block<T>(p, i, val, invd, bperm, xperm, tickets, s);
 }
 
 // Capture a block walk into a graph (once per plan and value type).
@@ -572,7 +720,144 @@ size_t blocked_ws_bytes(const rb_plan *p) {
     return 2 * sizeof(double) * (size_t)p->m + sizeof(int) * (size_t)p->ntri;
 }
 
-// FP32 solves refined in FP64 (rb_solve_f32ir): per right-hand side,
+// Kernel choice per block by timing.  Triangular blocks try the reference
+// selector's kernel, the sync-free grid kernel and, when every in-block
+// dependency is close enough for its ring, the one-workgroup kernel;
+// square blocks try powers of two from a quarter to four times the rule's
+// lanes per row (1 .. 64).  Each candidate runs
+// twice on the plan's own buffers (b = 0) and the faster time counts; the
+// block keeps the fastest candidate.  mode 1 keeps the reference
+// selector's kernels and the lanes-per-row rule; modes 2 / 3 put every
+// triangular block on the sync-free / one-workgroup kernel (where it
+// applies).  The captured block walk is dropped so the next solve captures
+// the chosen kernels.
+rb_status tune_blocks(rb_plan *p, int mode, hipStream_t s) {
+    const int nb = p->nblocks, ntri = p->ntri;
+    for (int i = 0; i < nb; i++) {
+        Block &B = p->blocks[i];
+        B.kern = kTriRef;
+        B.lanes = B.kind == 1 ? lanes_per_row(B.nnzr) : 0;
+        B.tuned_ms = 0.0;
+        B.maxdist = 0;
+    }
+    destroy_graph(p);
+    if (mode == 1 || p->m == 0) return RB_OK;
+    // in-block dependency distances of the triangular blocks
+    {
+        std::vector<int> hs(ntri), hm(ntri), hd(ntri, 0);
+        int mx = 1;
+        for (int t = 0; t < ntri; t++) {
+            hs[t] = p->blocks[2 * t].slot_base;
+            hm[t] = p->blocks[2 * t].blk_m;
+            mx = std::max(mx, hm[t]);
+        }
+        int *d = nullptr;
+        RB_HIP(hipMalloc(&d, sizeof(int) * 3 * (size_t)ntri));
+        rb_status st = RB_OK;
+        if (hipMemcpyAsync(d, hs.data(), sizeof(int) * ntri, hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipMemcpyAsync(d + ntri, hm.data(), sizeof(int) * ntri, hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipMemsetAsync(d + 2 * ntri, 0, sizeof(int) * ntri, s) != hipSuccess)
+            st = RB_ERR_HIP;
+        if (st == RB_OK) {
+            k_tri_maxdist<<<dim3(std::min(nblk(mx), 64), ntri), kT, 0, s>>>(
+                p->d_rowptr, p->d_col, d, d + ntri, d + 2 * ntri);
+            if (hipMemcpyAsync(hd.data(), d + 2 * ntri, sizeof(int) * ntri,
+                               hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess)
+                st = RB_ERR_HIP;
+        }
+        (void)hipFree(d);
+        if (st != RB_OK) return status_of_hip(hipGetLastError(), "tune_blocks");
+        for (int t = 0; t < ntri; t++) p->blocks[2 * t].maxdist = hd[t];
+    }
+    auto group_ok = [&](const Block &B) {
+        return B.maxdist <= kRingRows - kGroupThreads;
+    };
+    if (mode == 2 || mode == 3) {
+        for (int t = 0; t < ntri; t++) {
+            Block &B = p->blocks[2 * t];
+            if (B.method == 0 || B.blk_m == 0) continue;
+            if (mode == 2) B.kern = kTriSyncfree;
+            else if (group_ok(B)) B.kern = kTriGroup;
+        }
+        return RB_OK;
+    }
+    // mode 0: time the candidates
+    hipEvent_t e0, e1;
+    RB_HIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return status_of_hip(hipGetLastError(), "hipEventCreate");
+    }
+    rb_status st = RB_OK;
+    if (hipMemsetAsync(p->d_bperm, 0, sizeof(double) * (size_t)p->m, s) != hipSuccess ||
+        hipMemsetAsync(p->d_xperm, 0, sizeof(double) * (size_t)p->m, s) != hipSuccess)
+        st = RB_ERR_HIP;
+    auto time_block = [&](int i) -> double {
+        const Block &B = p->blocks[i];
+        double best = 1e30;
+        for (int rep = 0; rep < 2 && st == RB_OK; rep++) {
+            if (B.kind == 0)
+                k_block_reset<<<nblk(B.blk_m), kT, 0, s>>>(p->d_xperm + B.up, B.blk_m,
+                                                           p->d_tickets + i / 2);
+            if (hipEventRecord(e0, s) != hipSuccess) st = RB_ERR_HIP;
+            launch_block<double>(p, i, p->d_val, p->d_invd, p->d_bperm, p->d_xperm,
+                                 p->d_tickets, s);
+            float ms = 0.f;
+            if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+                st = RB_ERR_HIP;
+            best = std::min(best, (double)ms);
+        }
+        return best;
+    };
+    // blocks below this many entries keep the rule (their launches cost
+    // more than any difference between kernels)
+    constexpr int kTuneMinEntries = 1 << 14;
+    for (int i = 0; i < nb && st == RB_OK; i++) {
+        Block &B = p->blocks[i];
+        if (B.blk_m == 0 || B.blk_nnz < kTuneMinEntries) continue;
+        std::vector<std::pair<int, int>> cand;  // (kern, lanes)
+        if (B.kind == 0) {
+            if (B.method == 0) continue;
+            cand.push_back({kTriRef, 0});
+            if (B.method == 2) cand.push_back({kTriSyncfree, 0});
+            if (group_ok(B)) cand.push_back({kTriGroup, 0});
+        } else {
+            const int nr = (B.method == 1 || B.method == 3) ? B.m_new : B.blk_m;
+            if (B.method < 0 || nr == 0) continue;
+            // the rule's lanes per row and two powers of two either side
+            const int g0 = lanes_per_row(B.nnzr);
+            for (int g = std::max(1, g0 / 4); g <= std::min(64, g0 * 4); g <<= 1)
+                cand.push_back({0, g});
+        }
+        if (cand.size() < 2) continue;
+        double best = 1e30;
+        std::pair<int, int> pick = cand[0];
+        for (const auto &c : cand) {
+            B.kern = c.first;
+            if (B.kind == 1) B.lanes = c.second;
+            const double t = time_block(i);
+            if (t < best) {
+                best = t;
+                pick = c;
+            }
+        }
+        B.kern = pick.first;
+        if (B.kind == 1) B.lanes = pick.second;
+        B.tuned_ms = best;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    // a tripped watchdog during tuning would poison the plan's status
+    int z = 0;
+    if (st == RB_OK && hipMemcpy(p->d_status, &z, sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+        st = RB_ERR_HIP;
+    if (st != RB_OK) return status_of_hip(hipGetLastError(), "tune_blocks");
+    return RB_OK;
+}
+
+// FP32 solves refined in FP64 (rb_solve_f32ir)// FP32 solves refined in FP64 (rb_solve_f32ir): per right-hand side,
 // x = 0, r = b; repeat { d = solve32((float) r); x += d; r = b - A x (FP64) }
 // until max|r| <= tol * max|b| or max_iter corrections.
 rb_status solve_f32ir(rb_plan *p, const double *d_b, int ldb, double *d_x,

@@ -27,7 +27,7 @@ EXPORTS = [
     "rb_batch_create", "rb_batch_solve", "rb_batch_solve_f32ir",
     "rb_batch_plan", "rb_batch_destroy",
     "rb_mmio_info", "rb_mmio_data", "rb_mmio_triangle",
-    "rb_solve_noalloc", "rb_noalloc_limit",
+    "rb_solve_noalloc", "rb_noalloc_limit", "rb_plan_block_kernels",
 ]
 
 RB_OK = 0
@@ -50,7 +50,8 @@ class rb_opts(ctypes.Structure):
                 ("fused_waves", ctypes.c_int), ("no_graph", ctypes.c_int),
                 ("mixed", ctypes.c_int), ("tune_rhs", ctypes.c_int),
                 ("canonicalize", ctypes.c_int),
-                ("watchdog_spins", ctypes.c_int)]
+                ("watchdog_spins", ctypes.c_int),
+                ("block_kernels", ctypes.c_int)]
 
 
 class rb_plan_info(ctypes.Structure):
@@ -107,6 +108,8 @@ def lib():
     L.rb_plan_info_get.restype = i
     L.rb_plan_blocks.argtypes = [vp] + [vp] * 15
     L.rb_plan_blocks.restype = i
+    L.rb_plan_block_kernels.argtypes = [vp, vp, vp, vp]
+    L.rb_plan_block_kernels.restype = i
     L.rb_plan_levels.argtypes = [vp, vp, vp]
     L.rb_plan_levels.restype = i
     L.rb_plan_tri_levels.argtypes = [vp, i, vp, vp, vp]

@@ -97,7 +97,7 @@ class Plan:
     def __init__(self, colptr, rowidx, val, substitution=0, lv=-1,
                  schedule=_lib.SCHED_AUTO, stream=None, device_id=0,
                  fused_waves=0, graph=True, mixed=False, tune_rhs=1,
-                 canonicalize=False, watchdog_spins=0):
+                 canonicalize=False, watchdog_spins=0, block_kernels=0):
         torch = _torch()
         self._keep = []
         dcp = _dev_vec(colptr, torch.int32, what="colptr")
@@ -117,6 +117,7 @@ class Plan:
         opts.tune_rhs = tune_rhs
         opts.canonicalize = 1 if canonicalize else 0
         opts.watchdog_spins = int(watchdog_spins)
+        opts.block_kernels = int(block_kernels)
         h = ctypes.c_void_p()
         check(lib().rb_plan_create(_ptr(dcp), _ptr(dri), _ptr(dv), self.m,
                                    self.m, self.nnz, substitution, lv,
@@ -208,6 +209,18 @@ class Plan:
                                    er.ctypes.data))
         arrs["empty_ratio"] = er
         return arrs
+
+    def block_kernels(self):
+        """Kernels the blocked schedule runs per block: dict of kern
+        (triangular: 0 reference selector's, 1 sync-free grid, 2 one
+        workgroup), lanes (square: lanes per row) and tuned_ms."""
+        nb = self.nblocks
+        kern = np.zeros(nb, np.int32)
+        lanes = np.zeros(nb, np.int32)
+        ms = np.zeros(nb, np.float64)
+        check(lib().rb_plan_block_kernels(self._h, kern.ctypes.data,
+                                          lanes.ctypes.data, ms.ctypes.data))
+        return {"kern": kern, "lanes": lanes, "tuned_ms": ms}
 
     def levels(self):
         li = np.zeros(max(self.m, 1), np.int32)

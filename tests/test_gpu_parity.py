@@ -193,11 +193,12 @@ def test_multi_rhs_and_leading_dimension(oracle, torch, schedule, k):
 @pytest.mark.parametrize("schedule", [0, 1, 2])
 def test_plan_build_phases(torch, schedule):
     """The plan reports its build phases: non-negative, the first three
-    adding up to preprocess_ms; timed tuning solves only when the tuner
-    chooses, and no fused build when the blocked schedule is asked for
+    adding up to preprocess_ms; no timed tuning when the fused schedule is
+    asked for or the blocked schedule keeps the reference selector's
+    kernels, and no fused build when the blocked schedule is asked for
     (those phases are then two clock reads apart)."""
     f = gen.laplacian(60)
-    inf = _plan(f, 2, schedule=schedule).info()
+    inf = _plan(f, 2, schedule=schedule, block_kernels=1).info()
     ph = list(inf.phase_ms)
     assert all(t >= 0.0 for t in ph)
     assert abs(sum(ph[:3]) - inf.preprocess_ms) <= 1e-9 * max(1.0, inf.preprocess_ms)
