@@ -181,7 +181,9 @@ def bench_batch(args):
     bt = Batch(facs, lv=-1, schedule=args.schedule, tune_rhs=max(nrhs))
     pre_ms = (time.perf_counter() - t0) * 1e3
     info = bt.info()
-    say("union plan built: %.0f ms, schedule %d" % (pre_ms, info.schedule))
+    say("union plan built: %.0f ms, schedule %d (warm solve ms: blocked "
+        "%.3f, fused %.3f)" % (pre_ms, info.schedule, info.tune_ms[0],
+                               info.tune_ms[1]))
     ms64, xs = timed(lambda: bt.solve(bs), args.steps)
     say("batched fp64: %.3f ms per batch" % ms64)
     res64 = worst_residual(xs)
@@ -224,6 +226,8 @@ def bench_batch(args):
                    "nnz_total": int(sum(f.nnz for f in facs)),
                    "schedule": {1: "blocked", 2: "fused"}.get(info.schedule),
                    "levels": info.nlevel, "preprocess_ms": round(pre_ms, 1),
+                   "tune_ms": [round(info.tune_ms[0], 3),
+                               round(info.tune_ms[1], 3)],
                    "residual": res64},
         "gbs_algorithmic": round(B_alg / (ms64 * 1e-3) / 1e9, 3),
         "f32ir": {"solves_per_s": round(K / (ms32 * 1e-3), 1),
@@ -263,7 +267,8 @@ def bench_serve(args):
     for name, mode, mb in modes:
         t0 = time.perf_counter()
         srv = Server(facs, mode=mode, max_batch=mb, streams=4,
-                     mem_budget=8 << 30)
+                     mem_budget=8 << 30,
+                     log=lambda msg, name=name: say(name, msg))
         build_s = time.perf_counter() - t0
         say(name, "server ready in %.1f s" % build_s)
         stats, out = srv.run(reqs)

@@ -165,7 +165,9 @@ rb_status rb_plan_destroy(rb_plan *plan);
  * whose wait gave up until this call, every solve on the plan writes quiet
  * NaN to all of x (never a finite wrong answer); this call then clears the
  * flag and re-arms the plan's per-solve state, so later solves run
- * normally.  No counterpart in the reference, whose spin loops have no
+ * normally.  Otherwise RB_ERR_NOT_CONVERGED when an rb_solve_f32ir call
+ * without a report left a column short of its tolerance (the flag is
+ * cleared).  No counterpart in the reference, whose spin loops have no
  * bound. */
 rb_status rb_plan_check(rb_plan *plan, void *stream);
 
@@ -181,6 +183,9 @@ typedef struct {
                              analysis; reorder, partition, blocks, selector
                              and storage; fused schedule build; schedule
                              tuning (timed solves, not in preprocess_ms) */
+    double tune_ms[2];    /* warm solve time of the blocked / fused schedule
+                             at tune_rhs columns when RB_SCHED_AUTO timed
+                             both, else 0 */
 } rb_plan_info;
 
 rb_status rb_plan_info_get(const rb_plan *plan, rb_plan_info *info);
@@ -241,11 +246,16 @@ rb_status rb_plan_trace(const rb_plan *plan, unsigned long long *counters,
  * blocked schedule's kernels; x += d in FP64; r = b - A x in FP64 on the
  * device } until max_i |r_i| <= tol * max_i |b_i| or max_iter corrections.
  * FP64 is used only for the residual and the update of x.  d_b / d_x as in
- * rb_solve.  *iters receives the most corrections any column needed and
- * *relres the largest final max|r| / max|b| (either may be NULL).  The call
- * synchronises on stream once per correction (to test convergence).
- * Returns RB_ERR_NOT_CONVERGED when a column missed tol (x then holds the
- * last iterate).  No counterpart in the reference (FP64 only). */
+ * rb_solve.  The loop runs on the device, up to 32 columns at a time;
+ * each column stops correcting once it has converged.  With iters or
+ * relres given, *iters receives the most corrections any column needed and
+ * *relres the largest final max|r| / max|b|; the call synchronises on
+ * stream after the second correction and each later one (once for the
+ * usual two) and returns RB_ERR_NOT_CONVERGED when a column missed tol (x
+ * then holds the last iterate).  With both NULL the call does not
+ * synchronise: max_iter corrections are queued and a column that missed
+ * tol sets the plan's status, which rb_plan_check reports as
+ * RB_ERR_NOT_CONVERGED.  No counterpart in the reference (FP64 only). */
 rb_status rb_solve_f32ir(rb_plan *plan, const double *d_b, int ldb,
                          double *d_x, int ldx, int nrhs, double tol,
                          int max_iter, void *stream, int *iters,

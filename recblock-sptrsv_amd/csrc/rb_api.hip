@@ -257,11 +257,16 @@ rb_status rb_plan_check(rb_plan *p, void *stream) {
         // solve leaves slots of both x buffers un-armed)
         const int z = 0;
         RB_HIP(hipMemcpy(p->d_status, &z, sizeof(int), hipMemcpyHostToDevice));
-        fused_rearm(p);
-        set_error(RB_ERR_INTERNAL,
-                  "device watchdog: a solve waited on a value that never "
-                  "arrived (x was written as NaN by every solve since)");
-        return RB_ERR_INTERNAL;
+        if (st & 7) {
+            fused_rearm(p);
+            set_error(RB_ERR_INTERNAL,
+                      "device watchdog: a solve waited on a value that never "
+                      "arrived (x was written as NaN by every solve since)");
+            return RB_ERR_INTERNAL;
+        }
+        set_error(RB_ERR_NOT_CONVERGED,
+                  "an asynchronous FP32 refinement missed its tolerance");
+        return RB_ERR_NOT_CONVERGED;
     }
     return ok();
 }
@@ -325,6 +330,7 @@ rb_status rb_plan_destroy(rb_plan *p) {
     if (p->ev1) (void)hipEventDestroy(p->ev1);
     destroy_graph(p);
     fused_destroy(p);
+    free_solve_scratch(p);
     p->arena.release();
     delete p;
     return ok();
@@ -360,6 +366,8 @@ rb_status rb_plan_info_get(const rb_plan *p, rb_plan_info *info) {
     info->preprocess_ms = p->preprocess_ms;
     info->device_bytes = p->arena.bytes;
     for (int i = 0; i < 4; i++) info->phase_ms[i] = p->phase_ms[i];
+    info->tune_ms[0] = p->tune_ms_blocked;
+    info->tune_ms[1] = p->tune_ms_fused;
     return ok();
 }
 

@@ -566,7 +566,7 @@ __device__ __forceinline__ void resolve(const FusedArgs &a, int w, int r,
         } else {
             ++spins;
             if (spins > a.spin || lds_load_u(&sh.abort) ||
-                ((spins & 1023) == 0 && status_now(a.status))) {
+                ((spins & 1023) == 0 && (status_now(a.status) & 7))) {
                 if (lane == 0) {
                     atomicOr(a.status, 4);
                     lds_store(&sh.abort, 1);
@@ -684,7 +684,7 @@ __global__ void __launch_bounds__(kTileThreads)
     const int cnt = (int)min((long)kTile, nslots - base);
     const double *in = xq + blockIdx.y * nslots + base;
     // after a watchdog trip every x of the plan's solves is a quiet NaN
-    const bool tripped = *status != 0;
+    const bool tripped = (*status & 7) != 0;
 #pragma unroll
     for (int k = 0; k < kTile / kTileThreads; k++) {
         const int i = k * kTileThreads + threadIdx.x;

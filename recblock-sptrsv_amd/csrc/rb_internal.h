@@ -109,6 +109,7 @@ constexpr int kTriSyncfree = 1;  // sync-free CSR pull over the whole grid
 constexpr int kTriGroup = 2;     // one workgroup, x ring in LDS
 constexpr int kRingRows = 8192;  // kTriGroup's ring (positions)
 constexpr int kGroupThreads = 1024;
+constexpr int kGroupSlack = 2;   // kTriGroup: rounds a wave may run ahead
 
 struct FusedPlan;
 
@@ -155,6 +156,7 @@ struct rb_plan {
     double *d_xperm = nullptr;  // m
     int *d_tickets = nullptr;   // one per triangular block
     int *d_status = nullptr;    // device watchdog flags (0 = healthy)
+    int *d_ncol = nullptr;      // [0] columns of the running walk, [1] = 1
     unsigned spin_limit = 1u << 22;  // polls before a bounded wait gives up
     int scratch_rhs = 0;
 
@@ -164,9 +166,12 @@ struct rb_plan {
     int fused_waves = 0;        // wave cap for the fused partition (0 = auto)
     double tune_ms_blocked = 0.0, tune_ms_fused = 0.0;
     int tune_rhs = 1;           // right-hand sides per tuning solve
-    // blocked schedule: the block walk captured once as a graph
-    void *graph_exec = nullptr;
+    // blocked schedule: the block walk captured as graphs, per value type
+    // (FP64, FP32) and grid width (powers of two)
+    std::vector<std::pair<int, void *>> graphs[2];
     bool graph_failed = false;
+    // columns the permuted b / x (FP64, FP32) and tickets are sized for
+    int kcols = 1, kcols32 = 1, kticket = 1;
     // kernel timing (rb_plan_set_timing)
     bool timing = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -175,12 +180,15 @@ struct rb_plan {
     bool mixed = false;
     float *d_val32 = nullptr, *d_invd32 = nullptr;   // slot values in FP32
     float *d_bperm32 = nullptr, *d_xperm32 = nullptr;
-    void *graph_exec32 = nullptr;                    // FP32 block walk
     int *d_rptr = nullptr, *d_rcol = nullptr;        // FP64 matrix for the
     double *d_rval = nullptr;                        // residual (see below)
+    // refinement scratch for kir right-hand sides (allocated on first use)
+    int kir = 0;
     double *d_r = nullptr;                           // residual, FP64
     float *d_r32 = nullptr, *d_d32 = nullptr;        // residual / correction
     unsigned long long *d_norms = nullptr;           // max |b|, max |r|
+    double *d_irrel = nullptr;                       // relative residuals
+    int *d_irdone = nullptr, *d_iriters = nullptr;   // converged, corrections
 };
 
 namespace rb {
@@ -226,6 +234,10 @@ inline void mark_end(rb_plan *p, hipStream_t s) {
     if (p->timing) (void)hipEventRecord(p->ev1, s);
 }
 void destroy_graph(rb_plan *p);
+// frees the right-hand-side and refinement scratch grown beyond the plan's
+// own (one column)
+void free_solve_scratch(rb_plan *p);
+void free_ir(rb_plan *p);
 // after a watchdog trip: every slot-order x buffer of the fused schedule is
 // filled with the sentinel again before its next use
 void fused_rearm(rb_plan *p);

@@ -854,10 +854,6 @@ rb_status build_plan(rb_plan *p, const int *d_colptr, const int *d_rowidx,
         RB_HIP(hipGetLastError());
         RB_TRY(A.alloc(&p->d_bperm32, m));
         RB_TRY(A.alloc(&p->d_xperm32, m));
-        RB_TRY(A.alloc(&p->d_r, m));
-        RB_TRY(A.alloc(&p->d_r32, m));
-        RB_TRY(A.alloc(&p->d_d32, m));
-        RB_TRY(A.alloc(&p->d_norms, 2));
     }
 
     // ---- 11. solve scratch
@@ -866,6 +862,8 @@ rb_status build_plan(rb_plan *p, const int *d_colptr, const int *d_rowidx,
     RB_TRY(A.alloc(&p->d_tickets, ntri));
     RB_TRY(A.alloc(&p->d_status, 1));
     RB_HIP(hipMemsetAsync(p->d_status, 0, sizeof(int), s));
+    RB_TRY(A.alloc(&p->d_ncol, 2));
+    RB_HIP(hipMemsetD32Async((hipDeviceptr_t)p->d_ncol, 1, 2, s));
     RB_HIP(hipStreamSynchronize(s));
     auto t1 = std::chrono::steady_clock::now();
     p->preprocess_ms =

@@ -69,32 +69,46 @@ __device__ inline bool is_pending(T v) {
     return __builtin_bit_cast(typename Bits<T>::U, v) == Bits<T>::kPend;
 }
 
+// Right-hand side blockIdx.y of a walk over several: b and x in the
+// caller's layout (leading dimensions ldb, ldx), the permuted vectors and
+// tickets of column y at y * ld and y * ntri.
 template <class T>
-__global__ void k_prologue(const T *__restrict__ b, const int *__restrict__ lf,
-                           int m, T *bperm, T *xperm, int *tickets, int ntri) {
+__global__ void k_prologue(const T *__restrict__ b, long ldb,
+                           const int *__restrict__ lf, int m, long ld,
+                           T *bperm, T *xperm, int *tickets, int ntri,
+                           int *ncolw) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    // the walk's column count, read by its kernels (graphs are captured
+    // for a few grid widths, each solving the first *ncolw columns)
+    if (ncolw && i == 0 && c == 0) *ncolw = gridDim.y;
     if (i < m) {
-        bperm[i] = b[lf[i]];
-        ((typename Bits<T>::U *)xperm)[i] = Bits<T>::kPend;
+        bperm[c * ld + i] = b[c * ldb + lf[i]];
+        ((typename Bits<T>::U *)xperm)[c * ld + i] = Bits<T>::kPend;
     }
-    if (i < ntri) tickets[i] = 0;
+    if (i < ntri) tickets[c * ntri + i] = 0;
 }
 
 // x back to the caller's order; after a watchdog trip (status set) every
 // entry is a quiet NaN, so no solve of a tripped plan hands out a finite x
 template <class T>
-__global__ void k_epilogue(const T *__restrict__ xperm,
-                           const int *__restrict__ lf, int m, T *x,
+__global__ void k_epilogue(const T *__restrict__ xperm, long ld,
+                           const int *__restrict__ lf, int m, T *x, long ldx,
                            const int *status) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
     if (i < m)
-        x[lf[i]] = *status ? __builtin_bit_cast(T, Bits<T>::kQNaN) : xperm[i];
+        x[c * ldx + lf[i]] = (*status & 7) ? __builtin_bit_cast(T, Bits<T>::kQNaN)
+                                           : xperm[c * ld + i];
 }
 
 template <class T>
 __global__ void k_tri_diag(const T *__restrict__ invd,
                            const T *__restrict__ bperm, int up, int n,
-                           T *xperm) {
+                           T *xperm, long ld, const int *__restrict__ ncolp) {
+    if ((int)blockIdx.y >= *ncolp) return;
+    bperm += blockIdx.y * ld;
+    xperm += blockIdx.y * ld;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) st_x(&xperm[up + i], bperm[up + i] * invd[up + i]);
 }
@@ -107,7 +121,10 @@ __global__ void k_tri_level_thread(const int *__restrict__ rowptr,
                                    const T *__restrict__ invd,
                                    const T *__restrict__ bperm,
                                    const int *__restrict__ rows, int nrows,
-                                   int sb, int up, T *xperm) {
+                                   int sb, int up, T *xperm, long ld, const int *__restrict__ ncolp) {
+    if ((int)blockIdx.y >= *ncolp) return;
+    bperm += blockIdx.y * ld;
+    xperm += blockIdx.y * ld;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nrows) return;
     const int r = rows[k];
@@ -138,7 +155,10 @@ __global__ void k_tri_level_wave(const int *__restrict__ rowptr,
                                  const T *__restrict__ invd,
                                  const T *__restrict__ bperm,
                                  const int *__restrict__ rows, int nrows,
-                                 int sb, int up, T *xperm) {
+                                 int sb, int up, T *xperm, long ld, const int *__restrict__ ncolp) {
+    if ((int)blockIdx.y >= *ncolp) return;
+    bperm += blockIdx.y * ld;
+    xperm += blockIdx.y * ld;
     const int k = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     if (k >= nrows) return;
@@ -157,7 +177,11 @@ k_tri_syncfree(const int *__restrict__ rowptr, const int *__restrict__ col,
                const T *__restrict__ val, const T *__restrict__ invd,
                const T *__restrict__ bperm, int sb, int up, int bm,
                int subst, int *ticket, T *xperm, int *status,
-               unsigned spin_limit) {
+               unsigned spin_limit, long ld, int tstride, const int *__restrict__ ncolp) {
+    if ((int)blockIdx.y >= *ncolp) return;
+    bperm += blockIdx.y * ld;
+    xperm += blockIdx.y * ld;
+    ticket += blockIdx.y * tstride;
     const int lane = threadIdx.x & 63;
     int tile = 0;
     if (lane == 0) tile = atomicAdd(ticket, 1);
@@ -215,11 +239,16 @@ k_tri_syncfree(const int *__restrict__ rowptr, const int *__restrict__ col,
 // row polls only the ring -- a dependency is an LDS read, not a trip
 // through device memory as in the sync-free grid kernel, nor a launch per
 // level as in the level-set kernels -- so deep, narrow blocks (hundreds of
-// short levels) run faster.  Correct when every in-block dependency is at
-// most kRingRows - kGroupThreads positions back: all rows in flight lie
-// within 1024 positions of the oldest unsolved one, so a slot is reused
-// only after every row that reads it has finished (the host checks the
-// distance).  Lanes of one wave may depend on each other, so a wave loops
+// short levels) run faster.  The rows of thread t's k-th pass form round
+// k; a wave that is done with its round k moves on, but it starts round
+// k only once every wave is done with round k - kGroupSlack, and round
+// k writes the slots of positions kRingRows back.  So when every in-block
+// dependency is at most kRingRows - kGroupSlack * kGroupThreads positions
+// back (the host checks the distance), a slot is reused only after every
+// row that reads it has finished.  (Without the gate a wave whose rows
+// have no in-block dependencies would run rounds ahead and overwrite the
+// slots slower rows still wait for.)  Lanes of one wave may depend on
+// each other, so a wave loops
 // until all its lanes are done, each lane advancing as far as its
 // dependencies allow; a row's entries are read ahead in chunks of kChunk,
 // before the waits, and summed in order, as by the other kernels.
@@ -233,14 +262,37 @@ __global__ void __launch_bounds__(kGroupThreads)
 k_tri_group(const int *__restrict__ rowptr, const int *__restrict__ col,
             const T *__restrict__ val, const T *__restrict__ invd,
             const T *__restrict__ bperm, int sb, int up, int bm, int subst,
-            T *xperm, int *status, unsigned spin_limit) {
+            T *xperm, int *status, unsigned spin_limit, long ld, const int *__restrict__ ncolp) {
+    if ((int)blockIdx.y >= *ncolp) return;
+    bperm += blockIdx.y * ld;
+    xperm += blockIdx.y * ld;
     constexpr int kChunk = 8;
+    constexpr int kWaves = kGroupThreads / 64;
     __shared__ int tag[kRingRows];
     __shared__ T xr[kRingRows];
+    __shared__ int wround[kWaves];  // last round each wave has finished
     for (int i = threadIdx.x; i < kRingRows; i += kGroupThreads) tag[i] = -1;
+    if (threadIdx.x < kWaves) wround[threadIdx.x] = -1;
     __syncthreads();
     bool bad = false;
-    for (int p0 = threadIdx.x & ~63; p0 < bm; p0 += kGroupThreads) {
+    const int lane = threadIdx.x & 63;
+    int round = 0;
+    for (int p0 = threadIdx.x & ~63; p0 < bm; p0 += kGroupThreads, round++) {
+        // rounds before kRingRows / kGroupThreads reuse no slot
+        if (round >= kRingRows / kGroupThreads) {
+            const int need = round - kGroupSlack;
+            unsigned gspins = 0;
+            while (__any(lane < kWaves &&
+                         __hip_atomic_load(&wround[lane < kWaves ? lane : 0],
+                                           __ATOMIC_ACQUIRE,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP) < need)) {
+                if (++gspins > spin_limit) {  // flag it and go on
+                    bad = true;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
         const int pos = p0 + (threadIdx.x & 63);
         const bool valid = pos < bm;
         const int r = subst == 0 ? pos : bm - 1 - pos;
@@ -317,107 +369,70 @@ k_tri_group(const int *__restrict__ rowptr, const int *__restrict__ col,
                 }
             }
         }
+        if (lane == 0)
+            __hip_atomic_store(&wround[threadIdx.x >> 6], round, __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
     }
+    // a wave that has left holds no other wave back
+    if (lane == 0)
+        __hip_atomic_store(&wround[threadIdx.x >> 6], 0x7fffffff, __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
     if (bad) atomicOr(status, 1);
 }
 
-// square block SpMV, CSR, thread per row; b[up + r] -= A[r,:] x[left:]
-template <class T>
-__global__ void k_sq_csr_thread(const int *__restrict__ rowptr,
-                                const int *__restrict__ col,
-                                const T *__restrict__ val,
-                                const T *__restrict__ xperm, int sb,
-                                int up, int left, int bm, T *bperm) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= bm) return;
-    const int s = rowptr[sb + r], e = rowptr[sb + r + 1];
-    if (s == e || e - s > kLongRow) return;
-    T acc = 0;
-    for (int j = s; j < e; j++) acc += val[j] * xperm[left + col[j]];
-    bperm[up + r] -= acc;
-}
-
-template <class T>
-__global__ void k_sq_csr_wave(const int *__restrict__ rowptr,
-                              const int *__restrict__ col,
-                              const T *__restrict__ val,
-                              const T *__restrict__ xperm, int sb,
-                              int up, int left, int bm, T *bperm) {
-    const int r = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int lane = threadIdx.x & 63;
-    if (r >= bm) return;
-    const int s = rowptr[sb + r], e = rowptr[sb + r + 1];
-    if (s == e || e - s > kLongRow) return;
-    T acc = 0;
-    for (int j = s + lane; j < e; j += 64) acc += val[j] * xperm[left + col[j]];
-    acc = wave_sum(acc);
-    if (lane == 0) bperm[up + r] -= acc;
-}
-
-template <class T>
-__global__ void k_sq_dcsr_thread(const int *__restrict__ rowptr,
-                                 const int *__restrict__ col,
-                                 const T *__restrict__ val,
-                                 const T *__restrict__ xperm,
-                                 const int *__restrict__ rows, int nrows,
-                                 int sb, int up, int left, T *bperm) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nrows) return;
-    const int r = rows[k];
-    const int s = rowptr[sb + r], e = rowptr[sb + r + 1];
-    if (e - s > kLongRow) return;
-    T acc = 0;
-    for (int j = s; j < e; j++) acc += val[j] * xperm[left + col[j]];
-    bperm[up + r] -= acc;
-}
-
-template <class T>
-__global__ void k_sq_dcsr_wave(const int *__restrict__ rowptr,
-                               const int *__restrict__ col,
-                               const T *__restrict__ val,
-                               const T *__restrict__ xperm,
-                               const int *__restrict__ rows, int nrows,
-                               int sb, int up, int left, T *bperm) {
-    const int k = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int lane = threadIdx.x & 63;
-    if (k >= nrows) return;
-    const int r = rows[k];
-    const int s = rowptr[sb + r], e = rowptr[sb + r + 1];
-    if (e - s > kLongRow) return;
-    T acc = 0;
-    for (int j = s + lane; j < e; j += 64) acc += val[j] * xperm[left + col[j]];
-    acc = wave_sum(acc);
-    if (lane == 0) bperm[up + r] -= acc;
-}
-
-// square block SpMV with G lanes per row (G a power of two, 2..32): the
-// group reads its row's entries coalesced, then a fixed-order butterfly;
-// rows from the DCSR list (non-empty rows) or all rows of the block
-template <class T, int G, bool DCSR>
+// Square block SpMV, b[up + r] -= A[r, :] x[left:], with G lanes per row
+// (G a power of two, 1 .. 64): the group reads its row's entries
+// coalesced, then a fixed-order butterfly; rows from the DCSR list (the
+// non-empty rows) or all rows of the block; rows longer than 2048 entries
+// are left to k_sq_long.  Each thread keeps KC right-hand sides (columns
+// blockIdx.y * KC .. + KC - 1 of *ncolp, ld apart), so the block's entries
+// are read once per KC columns.
+template <class T, int G, bool DCSR, int KC>
 __global__ void __launch_bounds__(kT)
 k_sq_group(const int *__restrict__ rowptr, const int *__restrict__ col,
            const T *__restrict__ val, const T *__restrict__ xperm,
            const int *__restrict__ rows, int nrows, int sb, int up, int left,
-           T *bperm) {
+           T *bperm, long ld, const int *__restrict__ ncolp) {
+    const int c0 = blockIdx.y * KC;
+    const int ncol = *ncolp;
+    if (c0 >= ncol) return;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = t / G, g = t & (G - 1);
     if (k >= nrows) return;  // whole groups leave together
     const int r = DCSR ? rows[k] : k;
     const int s = rowptr[sb + r], e = rowptr[sb + r + 1];
     if (s == e || e - s > kLongRow) return;
-    T acc = 0;
-    for (int j = s + g; j < e; j += G) acc += val[j] * xperm[left + col[j]];
+    const int nc = min(KC, ncol - c0);
+    const T *x0 = xperm + c0 * ld + left;
+    T acc[KC];
 #pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, G);
-    if (g == 0) bperm[up + r] -= acc;
+    for (int c = 0; c < KC; c++) acc[c] = 0;
+    for (int j = s + g; j < e; j += G) {
+        const T v = val[j];
+        const int cj = col[j];
+#pragma unroll
+        for (int c = 0; c < KC; c++)
+            if (c < nc) acc[c] += v * x0[c * ld + cj];
+    }
+#pragma unroll
+    for (int c = 0; c < KC; c++) {
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) acc[c] += __shfl_xor(acc[c], o, G);
+        if (g == 0 && c < nc) bperm[(c0 + c) * ld + up + r] -= acc[c];
+    }
 }
 
-// one workgroup per long row (> 2048 entries), fixed-order tree reduction
+// one workgroup per long row (> 2048 entries) and right-hand side
+// (blockIdx.y), fixed-order tree reduction
 template <class T>
 __global__ void __launch_bounds__(kT)
 k_sq_long(const int *__restrict__ rowptr, const int *__restrict__ col,
           const T *__restrict__ val, const T *__restrict__ xperm,
-          const int *__restrict__ rows, int sb, int up, int left, T *bperm) {
+          const int *__restrict__ rows, int sb, int up, int left, T *bperm,
+          long ld, const int *__restrict__ ncolp) {
+    if ((int)blockIdx.y >= *ncolp) return;
+    xperm += blockIdx.y * ld;
+    bperm += blockIdx.y * ld;
     const int r = rows[blockIdx.x];
     const int s = rowptr[sb + r], e = rowptr[sb + r + 1];
     T acc = 0;
@@ -444,67 +459,139 @@ inline int lanes_per_row(int nnzr) {
     return g;
 }
 
-template <class T, bool DCSR>
-void launch_sq(int g, int nrows, hipStream_t s, const int *rowptr,
-               const int *col, const T *val, const T *xperm, const int *rows,
-               int sb, int up, int left, T *bperm) {
+// columns per thread of the SpMV for ncol right-hand sides
+inline int cols_per_thread(int ncol) { return ncol >= 4 ? 4 : ncol >= 2 ? 2 : 1; }
+
+template <class T, bool DCSR, int KC>
+void launch_sq_kc(int g, int nrows, int ncol, hipStream_t s, const int *rowptr,
+                  const int *col, const T *val, const T *xperm, const int *rows,
+                  int sb, int up, int left, T *bperm, long ld, const int *ncolp) {
     const long th = (long)nrows * g;
+    const dim3 grid(nblk(th), (ncol + KC - 1) / KC);
+#define RB_SQ(G) k_sq_group<T, G, DCSR, KC><<<grid, kT, 0, s>>>(rowptr, col, val, xperm, rows, nrows, sb, up, left, bperm, ld, ncolp)
     switch (g) {
-    case 2: k_sq_group<T, 2, DCSR><<<nblk(th), kT, 0, s>>>(rowptr, col, val, xperm, rows, nrows, sb, up, left, bperm); break;
-    case 4: k_sq_group<T, 4, DCSR><<<nblk(th), kT, 0, s>>>(rowptr, col, val, xperm, rows, nrows, sb, up, left, bperm); break;
-    case 8: k_sq_group<T, 8, DCSR><<<nblk(th), kT, 0, s>>>(rowptr, col, val, xperm, rows, nrows, sb, up, left, bperm); break;
-    case 16: k_sq_group<T, 16, DCSR><<<nblk(th), kT, 0, s>>>(rowptr, col, val, xperm, rows, nrows, sb, up, left, bperm); break;
-    default: k_sq_group<T, 32, DCSR><<<nblk(th), kT, 0, s>>>(rowptr, col, val, xperm, rows, nrows, sb, up, left, bperm); break;
+    case 1: RB_SQ(1); break;
+    case 2: RB_SQ(2); break;
+    case 4: RB_SQ(4); break;
+    case 8: RB_SQ(8); break;
+    case 16: RB_SQ(16); break;
+    case 32: RB_SQ(32); break;
+    default: RB_SQ(64); break;
+    }
+#undef RB_SQ
+}
+
+template <class T, bool DCSR>
+void launch_sq(int g, int nrows, int ncol, hipStream_t s, const int *rowptr,
+               const int *col, const T *val, const T *xperm, const int *rows,
+               int sb, int up, int left, T *bperm, long ld, const int *ncolp) {
+    switch (cols_per_thread(ncol)) {
+    case 4: launch_sq_kc<T, DCSR, 4>(g, nrows, ncol, s, rowptr, col, val, xperm, rows, sb, up, left, bperm, ld, ncolp); break;
+    case 2: launch_sq_kc<T, DCSR, 2>(g, nrows, ncol, s, rowptr, col, val, xperm, rows, sb, up, left, bperm, ld, ncolp); break;
+    default: launch_sq_kc<T, DCSR, 1>(g, nrows, ncol, s, rowptr, col, val, xperm, rows, sb, up, left, bperm, ld, ncolp); break;
     }
 }
 
 // ---------------------------------------------------- refinement kernels
-__global__ void k_to_f32(const double *__restrict__ a, int m, float *o) {
+// Several right-hand sides at once: column c = blockIdx.y; the refinement
+// scratch (r, r32, d32) holds column c at c * m; per column, bmax / rmax
+// hold max |b| / max |r| as the bits of a non-negative double (ordered like
+// the values), done[c] is set once the column has converged, and the
+// columns that are done skip every later pass.
+
+// r = b, x = 0, max |b|
+__global__ void k_ir_init(const double *__restrict__ b, long ldb, double *x,
+                          long ldx, double *r, int m, unsigned long long *bmax) {
+    const int c = blockIdx.y;
+    double mx = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m;
+         i += gridDim.x * blockDim.x) {
+        const double v = b[c * ldb + i];
+        r[(long)c * m + i] = v;
+        x[c * ldx + i] = 0.0;
+        mx = fmax(mx, fabs(v));
+    }
+    mx = wave_max(mx);
+    if ((threadIdx.x & 63) == 0)
+        atomicMax(&bmax[c], (unsigned long long)__double_as_longlong(mx));
+}
+
+__global__ void k_ir_to32(const double *__restrict__ r, int m, float *r32,
+                          const int *__restrict__ done) {
+    const int c = blockIdx.y;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < m) o[i] = (float)a[i];
+    if (!done[c] && i < m) r32[(long)c * m + i] = (float)r[(long)c * m + i];
 }
 
 // x += d (FP64 update with the FP32 correction)
-__global__ void k_add_f32(const float *__restrict__ d, int m, double *x) {
+__global__ void k_ir_update(const float *__restrict__ d, int m, double *x,
+                            long ldx, const int *__restrict__ done) {
+    const int c = blockIdx.y;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < m) x[i] += (double)d[i];
-}
-
-// max |v| over the vector, as the bits of a non-negative double (ordered
-// like the values) into *out (cleared by the caller)
-__global__ void k_absmax(const double *__restrict__ v, int m,
-                         unsigned long long *out) {
-    double mx = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m;
-         i += gridDim.x * blockDim.x)
-        mx = fmax(mx, fabs(v[i]));
-    mx = wave_max(mx);
-    if ((threadIdx.x & 63) == 0)
-        atomicMax(out, (unsigned long long)__double_as_longlong(mx));
+    if (!done[c] && i < m) x[c * ldx + i] += (double)d[(long)c * m + i];
 }
 
 // r = b - A x in the original ordering, FP64: row i of the level-ordered
-// CSR is original row lf[i], its columns hold original ids; also max |r|
-__global__ void k_residual(const int *__restrict__ rptr,
-                           const int *__restrict__ rcol,
-                           const double *__restrict__ rval,
-                           const int *__restrict__ lf, int m,
-                           const double *__restrict__ b,
-                           const double *__restrict__ x, double *r,
-                           unsigned long long *rmax) {
+// CSR is original row lf[i], its columns hold original ids; max |r|
+__global__ void k_ir_residual(const int *__restrict__ rptr,
+                              const int *__restrict__ rcol,
+                              const double *__restrict__ rval,
+                              const int *__restrict__ lf, int m,
+                              const double *__restrict__ b, long ldb,
+                              const double *__restrict__ x, long ldx,
+                              double *r, unsigned long long *rmax,
+                              const int *__restrict__ done) {
+    const int c = blockIdx.y;
+    if (done[c]) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double a = 0.0;
     if (i < m) {
         const int o = lf[i];
+        const double *xc = x + c * ldx;
         double acc = 0.0;
-        for (int k = rptr[i]; k < rptr[i + 1]; k++) acc += rval[k] * x[rcol[k]];
-        const double v = b[o] - acc;
-        r[o] = v;
+        for (int k = rptr[i]; k < rptr[i + 1]; k++) acc += rval[k] * xc[rcol[k]];
+        const double v = b[c * ldb + o] - acc;
+        r[(long)c * m + o] = v;
         a = fabs(v);
     }
     a = wave_max(a);
     if ((threadIdx.x & 63) == 0)
-        atomicMax(rmax, (unsigned long long)__double_as_longlong(a));
+        atomicMax(&rmax[c], (unsigned long long)__double_as_longlong(a));
+}
+
+// after correction `it`: a column whose max |r| <= tol * max |b| is done;
+// its relative residual and correction count are kept, rmax is cleared
+// for the next residual
+__global__ void k_ir_check(const unsigned long long *__restrict__ bmax,
+                           unsigned long long *rmax, int ncol, double tol,
+                           int *done, int *iters, double *rel, int it) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncol || done[c]) return;
+    const double bn = __longlong_as_double((long long)bmax[c]);
+    const double rn = __longlong_as_double((long long)rmax[c]);
+    const double q = bn > 0.0 ? rn / bn : 0.0;
+    rel[c] = q;
+    iters[c] = it;
+    if (q <= tol) done[c] = 1;
+    rmax[c] = 0;
+}
+
+__global__ void k_ir_reset(unsigned long long *norms, int *done, int *iters,
+                           double *rel, int ncol) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < ncol) {
+        norms[c] = 0;
+        norms[ncol + c] = 0;
+        done[c] = 0;
+        iters[c] = 0;
+        rel[c] = 0.0;
+    }
+}
+
+// asynchronous calls: a column that missed tol marks the plan (bit 8)
+__global__ void k_ir_flag(const int *__restrict__ done, int ncol, int *status) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < ncol && !done[c]) atomicOr(status, 8);
 }
 
 // largest in-block dependency distance |row - column| of every triangular
@@ -534,20 +621,23 @@ __global__ void k_block_reset(double *x, int n, int *ticket) {
 
 }  // namespace
 
-// One block of the walk in value type T (the kernels of B.kern / B.lanes).
+// One block of the walk in value type T (the kernels of B.kern / B.lanes)
+// on a grid of ncol columns, solving the first *nc of them: column c of
+// the permuted b and x at c * ld, its tickets at c * ntri.
 template <class T>
 static void launch_block(rb_plan *p, int i, const T *val, const T *invd,
-                         T *bperm, T *xperm, int *tickets, hipStream_t s) {
+                         T *bperm, T *xperm, int *tickets, int ncol, long ld,
+                         const int *nc, hipStream_t s) {
     const Block &B = p->blocks[i];
-    if (B.blk_m == 0) return;
+    if (B.blk_m == 0 || ncol <= 0) return;
     if (B.kind == 0) {
         if (B.method == 0) {
-            k_tri_diag<T><<<nblk(B.blk_m), kT, 0, s>>>(invd, bperm, B.up,
-                                                       B.blk_m, xperm);
+            k_tri_diag<T><<<dim3(nblk(B.blk_m), ncol), kT, 0, s>>>(
+                invd, bperm, B.up, B.blk_m, xperm, ld, nc);
         } else if (B.kern == kTriGroup) {
-            k_tri_group<T><<<1, kGroupThreads, 0, s>>>(
+            k_tri_group<T><<<dim3(1, ncol), kGroupThreads, 0, s>>>(
                 p->d_rowptr, p->d_col, val, invd, bperm, B.slot_base, B.up,
-                B.blk_m, p->subst, xperm, p->d_status, p->spin_limit);
+                B.blk_m, p->subst, xperm, p->d_status, p->spin_limit, ld, nc);
         } else if (B.kern == kTriRef && B.method == 2) {
             for (int L = 0; L < B.nlv; L++) {
                 const int a = B.lvptr_h[L], c = B.lvptr_h[L + 1];
@@ -555,19 +645,19 @@ static void launch_block(rb_plan *p, int i, const T *val, const T *invd,
                 if (nr <= 0) continue;
                 const int *rows = p->d_lvitem + B.lv_base + a;
                 if (L == 0 || B.nnzlv_h[L] / nr <= 15)
-                    k_tri_level_thread<T><<<nblk(nr), kT, 0, s>>>(
+                    k_tri_level_thread<T><<<dim3(nblk(nr), ncol), kT, 0, s>>>(
                         p->d_rowptr, p->d_col, val, invd, bperm, rows, nr,
-                        B.slot_base, B.up, xperm);
+                        B.slot_base, B.up, xperm, ld, nc);
                 else
-                    k_tri_level_wave<T><<<nblk((long)nr * 64), kT, 0, s>>>(
+                    k_tri_level_wave<T><<<dim3(nblk((long)nr * 64), ncol), kT, 0, s>>>(
                         p->d_rowptr, p->d_col, val, invd, bperm, rows, nr,
-                        B.slot_base, B.up, xperm);
+                        B.slot_base, B.up, xperm, ld, nc);
             }
         } else {  // methods 1 and 3, or kTriSyncfree
-            k_tri_syncfree<T><<<nblk((long)B.blk_m, 64), 64, 0, s>>>(
+            k_tri_syncfree<T><<<dim3(nblk((long)B.blk_m, 64), ncol), 64, 0, s>>>(
                 p->d_rowptr, p->d_col, val, invd, bperm, B.slot_base,
                 B.up, B.blk_m, p->subst, tickets + i / 2, xperm,
-                p->d_status, p->spin_limit);
+                p->d_status, p->spin_limit, ld, p->ntri, nc);
         }
         return;
     }
@@ -577,63 +667,58 @@ static void launch_block(rb_plan *p, int i, const T *val, const T *invd,
     const int *rows = p->d_dcsr_rows + B.dcsr_base;
     const bool dcsr = B.method == 1 || B.method == 3;
     const int nr = dcsr ? B.m_new : B.blk_m;
-    const int g = B.lanes;
     if (B.method < 0 || nr == 0) return;  // empty block
-    if (g == 1) {
-        if (dcsr)
-            k_sq_dcsr_thread<T><<<nblk(nr), kT, 0, s>>>(
-                p->d_rowptr, p->d_col, val, xperm, rows, nr, B.slot_base,
-                B.up, B.left, bperm);
-        else
-            k_sq_csr_thread<T><<<nblk(nr), kT, 0, s>>>(
-                p->d_rowptr, p->d_col, val, xperm, B.slot_base, B.up, B.left,
-                nr, bperm);
-    } else if (g == 64) {
-        if (dcsr)
-            k_sq_dcsr_wave<T><<<nblk((long)nr * 64), kT, 0, s>>>(
-                p->d_rowptr, p->d_col, val, xperm, rows, nr, B.slot_base,
-                B.up, B.left, bperm);
-        else
-            k_sq_csr_wave<T><<<nblk((long)nr * 64), kT, 0, s>>>(
-                p->d_rowptr, p->d_col, val, xperm, B.slot_base, B.up, B.left,
-                nr, bperm);
-    } else if (dcsr) {
-        launch_sq<T, true>(g, nr, s, p->d_rowptr, p->d_col, val, xperm, rows,
-                           B.slot_base, B.up, B.left, bperm);
-    } else {
-        launch_sq<T, false>(g, nr, s, p->d_rowptr, p->d_col, val, xperm,
-                            rows, B.slot_base, B.up, B.left, bperm);
-    }
+    if (dcsr)
+        launch_sq<T, true>(B.lanes, nr, ncol, s, p->d_rowptr, p->d_col, val,
+                           xperm, rows, B.slot_base, B.up, B.left, bperm, ld, nc);
+    else
+        launch_sq<T, false>(B.lanes, nr, ncol, s, p->d_rowptr, p->d_col, val,
+                            xperm, rows, B.slot_base, B.up, B.left, bperm, ld, nc);
     if (B.longrow > 0)
-        k_sq_long<T><<<B.longrow, kT, 0, s>>>(
+        k_sq_long<T><<<dim3(B.longrow, ncol), kT, 0, s>>>(
             p->d_rowptr, p->d_col, val, xperm, p->d_long_rows + B.long_base,
-            B.slot_base, B.up, B.left, bperm);
+            B.slot_base, B.up, B.left, bperm, ld, nc);
 }
 
 // The block walk of one solve in value type T: kernels on plan-owned
 // buffers only, so it can be captured once into a graph and replayed.
 template <class T>
 static void launch_blocks(rb_plan *p, const T *val, const T *invd, T *bperm,
-                          T *xperm, int *tickets, hipStream_t s) {
+                          T *xperm, int *tickets, int ncol, long ld,
+                          const int *nc, hipStream_t s) {
     for (int i = 0; i < p->nblocks; i++)
-        launch_block<T>(p, i, val, invd, bperm, xperm, tickets, s);
+        launch_block<T>(p, i, val, invd, bperm, xperm, tickets, ncol, ld, nc, s);
 }
 
-// Capture a block walk into a graph (once per plan and value type).
+// Capture the walk on a grid of ncol columns (a power of two) into a
+// graph, once per value type and width; a replay solves the first
+// *d_ncol columns, which the prologue sets.  The scratch the graphs hold
+// grows by powers of two (at most log2 of the column cap times), so a
+// stream of solves of varying widths replays at most six graphs per type.
 // Returns false when capture is not possible; the walk is then launched
 // kernel by kernel.
 template <class T>
-static bool capture_blocks(rb_plan *p, void **exec, const T *val,
-                           const T *invd, T *bperm, T *xperm) {
+static bool capture_blocks(rb_plan *p, int ncol, void **exec) {
+    constexpr int ty = sizeof(T) == 8 ? 0 : 1;
+    *exec = nullptr;
     if (p->graph_failed) return false;
-    if (*exec) return true;
+    for (auto &e : p->graphs[ty])
+        if (e.first == ncol) {
+            *exec = e.second;
+            return true;
+        }
+    const T *val = ty == 0 ? (const T *)p->d_val : (const T *)p->d_val32;
+    const T *invd = ty == 0 ? (const T *)p->d_invd : (const T *)p->d_invd32;
+    T *bperm = ty == 0 ? (T *)p->d_bperm : (T *)p->d_bperm32;
+    T *xperm = ty == 0 ? (T *)p->d_xperm : (T *)p->d_xperm32;
     hipStream_t cs = nullptr;
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
     bool ok = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) == hipSuccess;
     if (ok) ok = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) == hipSuccess;
     if (ok) {
-        launch_blocks<T>(p, val, invd, bperm, xperm, p->d_tickets, cs);
+        launch_blocks<T>(p, val, invd, bperm, xperm, p->d_tickets, ncol, p->m,
+                         p->d_ncol, cs);
         ok = hipStreamEndCapture(cs, &g) == hipSuccess && g != nullptr;
     }
     if (ok) ok = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) == hipSuccess;
@@ -644,33 +729,117 @@ static bool capture_blocks(rb_plan *p, void **exec, const T *val,
         p->graph_failed = true;
         return false;
     }
+    p->graphs[ty].push_back({ncol, (void *)ge});
     *exec = ge;
     return true;
 }
 
+void free_solve_scratch(rb_plan *p) {
+    if (p->kcols > 1) {
+        (void)hipFree(p->d_bperm);
+        (void)hipFree(p->d_xperm);
+    }
+    if (p->kcols32 > 1) {
+        (void)hipFree(p->d_bperm32);
+        (void)hipFree(p->d_xperm32);
+    }
+    if (p->kticket > 1) (void)hipFree(p->d_tickets);
+    p->kcols = p->kcols32 = p->kticket = 1;
+    free_ir(p);
+}
+
+static void drop_graphs(rb_plan *p, int ty) {
+    for (auto &e : p->graphs[ty]) (void)hipGraphExecDestroy((hipGraphExec_t)e.second);
+    p->graphs[ty].clear();
+}
+
 void destroy_graph(rb_plan *p) {
-    if (p->graph_exec) (void)hipGraphExecDestroy((hipGraphExec_t)p->graph_exec);
-    if (p->graph_exec32) (void)hipGraphExecDestroy((hipGraphExec_t)p->graph_exec32);
-    p->graph_exec = nullptr;
-    p->graph_exec32 = nullptr;
+    drop_graphs(p, 0);
+    drop_graphs(p, 1);
+}
+
+// Right-hand sides one walk takes: up to 32, within about 1 GiB of
+// permuted b and x.
+static int walk_cols(const rb_plan *p) {
+    const long per = 16L * std::max(1, p->m);
+    return (int)std::max(1L, std::min(32L, (1L << 30) / per));
+}
+
+// Grid width of a walk of ncol columns: the next power of two, at most
+// the cap (itself sized for the scratch).
+static int walk_width(const rb_plan *p, int ncol) {
+    int w = 1;
+    while (w < ncol) w *= 2;
+    return std::min(w, walk_cols(p));
+}
+
+// Permuted b / x and tickets for ncol right-hand sides (value type T);
+// growing them drops the graphs that hold the old buffers.
+template <class T>
+static rb_status ensure_cols(rb_plan *p, int ncol, hipStream_t s) {
+    constexpr int ty = sizeof(T) == 8 ? 0 : 1;
+    int &cap = ty == 0 ? p->kcols : p->kcols32;
+    if (ncol <= cap) return RB_OK;
+    RB_HIP(hipStreamSynchronize(s));
+    T *&bp = ty == 0 ? (T *&)p->d_bperm : (T *&)p->d_bperm32;
+    T *&xp = ty == 0 ? (T *&)p->d_xperm : (T *&)p->d_xperm32;
+    const size_t n = (size_t)ncol * p->m;
+    T *nb = nullptr, *nx = nullptr;
+    int *nt = nullptr;
+    if (hipMalloc(&nb, sizeof(T) * n) != hipSuccess ||
+        hipMalloc(&nx, sizeof(T) * n) != hipSuccess ||
+        (ncol > p->kticket &&
+         hipMalloc(&nt, sizeof(int) * (size_t)ncol * std::max(1, p->ntri)) != hipSuccess)) {
+        (void)hipGetLastError();
+        if (nb) (void)hipFree(nb);
+        if (nx) (void)hipFree(nx);
+        set_error(RB_ERR_NO_MEMORY, "hipMalloc failed (right-hand side scratch)");
+        return RB_ERR_NO_MEMORY;
+    }
+    drop_graphs(p, 0);
+    drop_graphs(p, 1);
+    // the first buffers belong to the plan's arena; later ones to the plan
+    if (cap > 1) {
+        (void)hipFree(bp);
+        (void)hipFree(xp);
+    }
+    bp = nb;
+    xp = nx;
+    if (nt) {
+        if (p->kticket > 1) (void)hipFree(p->d_tickets);
+        p->d_tickets = nt;
+        p->kticket = ncol;
+    }
+    cap = ncol;
+    return RB_OK;
 }
 
 template <class T>
-static rb_status walk(rb_plan *p, void **exec, const T *val, const T *invd,
-                      T *bperm, T *xperm, const T *b, T *x, hipStream_t s) {
+static rb_status walk(rb_plan *p, int ncol, const T *b, long ldb, T *x,
+                      long ldx, hipStream_t s) {
+    constexpr int ty = sizeof(T) == 8 ? 0 : 1;
     const int m = p->m;
-    const bool graph = capture_blocks<T>(p, exec, val, invd, bperm, xperm);
+    const int width = walk_width(p, ncol);
+    RB_TRY(ensure_cols<T>(p, width, s));
+    const T *val = ty == 0 ? (const T *)p->d_val : (const T *)p->d_val32;
+    const T *invd = ty == 0 ? (const T *)p->d_invd : (const T *)p->d_invd32;
+    T *bperm = ty == 0 ? (T *)p->d_bperm : (T *)p->d_bperm32;
+    T *xperm = ty == 0 ? (T *)p->d_xperm : (T *)p->d_xperm32;
+    void *exec = nullptr;
+    const bool graph = capture_blocks<T>(p, width, &exec);
     const int pro = nblk(m > p->ntri ? m : p->ntri);
-    k_prologue<T><<<pro, kT, 0, s>>>(b, p->d_levelItem, m, bperm, xperm,
-                                     p->d_tickets, p->ntri);
+    k_prologue<T><<<dim3(pro, ncol), kT, 0, s>>>(b, ldb, p->d_levelItem, m, m,
+                                                 bperm, xperm, p->d_tickets,
+                                                 p->ntri, p->d_ncol);
     mark_start(p, s);
     if (graph)
-        RB_HIP(hipGraphLaunch((hipGraphExec_t)*exec, s));
+        RB_HIP(hipGraphLaunch((hipGraphExec_t)exec, s));
     else
-        launch_blocks<T>(p, val, invd, bperm, xperm, p->d_tickets, s);
+        launch_blocks<T>(p, val, invd, bperm, xperm, p->d_tickets, width, m,
+                         p->d_ncol, s);
     mark_end(p, s);
-    k_epilogue<T><<<nblk(m), kT, 0, s>>>(xperm, p->d_levelItem, m, x,
-                                         p->d_status);
+    k_epilogue<T><<<dim3(nblk(m), ncol), kT, 0, s>>>(xperm, m, p->d_levelItem,
+                                                     m, x, ldx, p->d_status);
     RB_HIP(hipGetLastError());
     return RB_OK;
 }
@@ -678,16 +847,17 @@ static rb_status walk(rb_plan *p, void **exec, const T *val, const T *invd,
 rb_status solve_blocked(rb_plan *p, const double *d_b, int ldb, double *d_x,
                         int ldx, int nrhs, hipStream_t s) {
     if (p->m == 0) return RB_OK;
-    for (int k = 0; k < nrhs; k++)
-        RB_TRY(walk<double>(p, &p->graph_exec, p->d_val, p->d_invd,
-                            p->d_bperm, p->d_xperm, d_b + (size_t)k * ldb,
-                            d_x + (size_t)k * ldx, s));
+    const int cap = walk_cols(p);
+    for (int k = 0; k < nrhs; k += cap)
+        RB_TRY(walk<double>(p, std::min(cap, nrhs - k), d_b + (size_t)k * ldb,
+                            ldb, d_x + (size_t)k * ldx, ldx, s));
     return RB_OK;
 }
 
 // The block walk on caller-provided per-solve buffers, launched kernel by
 // kernel (a captured graph holds the plan's own buffers): ws holds the
-// permuted b and x (m values each) and the triangular blocks' tickets.
+// permuted b and x (m values each) and the triangular blocks' tickets;
+// one right-hand side at a time.
 rb_status solve_blocked_ws(rb_plan *p, const double *d_b, int ldb,
                            double *d_x, int ldx, int nrhs, hipStream_t s,
                            void *ws, size_t ws_bytes) {
@@ -702,15 +872,16 @@ rb_status solve_blocked_ws(rb_plan *p, const double *d_b, int ldb,
     int *tickets = (int *)(xperm + m);
     for (int k = 0; k < nrhs; k++) {
         const int pro = nblk(m > p->ntri ? m : p->ntri);
-        k_prologue<double><<<pro, kT, 0, s>>>(d_b + (size_t)k * ldb,
-                                              p->d_levelItem, m, bperm, xperm,
-                                              tickets, p->ntri);
+        k_prologue<double><<<pro, kT, 0, s>>>(d_b + (size_t)k * ldb, ldb,
+                                              p->d_levelItem, m, m, bperm,
+                                              xperm, tickets, p->ntri, nullptr);
         mark_start(p, s);
         launch_blocks<double>(p, p->d_val, p->d_invd, bperm, xperm, tickets,
-                              s);
+                              1, m, p->d_ncol + 1, s);
         mark_end(p, s);
         k_epilogue<double><<<nblk(m), kT, 0, s>>>(
-            xperm, p->d_levelItem, m, d_x + (size_t)k * ldx, p->d_status);
+            xperm, m, p->d_levelItem, m, d_x + (size_t)k * ldx, ldx,
+            p->d_status);
         RB_HIP(hipGetLastError());
     }
     return RB_OK;
@@ -771,7 +942,7 @@ rb_status tune_blocks(rb_plan *p, int mode, hipStream_t s) {
         for (int t = 0; t < ntri; t++) p->blocks[2 * t].maxdist = hd[t];
     }
     auto group_ok = [&](const Block &B) {
-        return B.maxdist <= kRingRows - kGroupThreads;
+        return B.maxdist <= kRingRows - kGroupSlack * kGroupThreads;
     };
     if (mode == 2 || mode == 3) {
         for (int t = 0; t < ntri; t++) {
@@ -802,12 +973,24 @@ rb_status tune_blocks(rb_plan *p, int mode, hipStream_t s) {
                                                            p->d_tickets + i / 2);
             if (hipEventRecord(e0, s) != hipSuccess) st = RB_ERR_HIP;
             launch_block<double>(p, i, p->d_val, p->d_invd, p->d_bperm, p->d_xperm,
-                                 p->d_tickets, s);
+                                 p->d_tickets, 1, p->m, p->d_ncol + 1, s);
             float ms = 0.f;
             if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                 hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
                 st = RB_ERR_HIP;
             best = std::min(best, (double)ms);
+        }
+        // a candidate whose bounded waits gave up is never picked (the
+        // flag is cleared for the next one)
+        int trip = 0;
+        if (st == RB_OK &&
+            hipMemcpy(&trip, p->d_status, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+            st = RB_ERR_HIP;
+        if (trip != 0) {
+            const int z = 0;
+            if (hipMemcpy(p->d_status, &z, sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+                st = RB_ERR_HIP;
+            return 2e30;
         }
         return best;
     };
@@ -832,7 +1015,7 @@ rb_status tune_blocks(rb_plan *p, int mode, hipStream_t s) {
                 cand.push_back({0, g});
         }
         if (cand.size() < 2) continue;
-        double best = 1e30;
+        double best = 1e30;  // a tripped candidate times 2e30
         std::pair<int, int> pick = cand[0];
         for (const auto &c : cand) {
             B.kern = c.first;
@@ -857,53 +1040,118 @@ rb_status tune_blocks(rb_plan *p, int mode, hipStream_t s) {
     return RB_OK;
 }
 
-// FP32 solves refined in FP64 (rb_solve_f32ir)// FP32 solves refined in FP64 (rb_solve_f32ir): per right-hand side,
-// x = 0, r = b; repeat { d = solve32((float) r); x += d; r = b - A x (FP64) }
-// until max|r| <= tol * max|b| or max_iter corrections.
+// Refinement scratch for ncol right-hand sides: r (FP64), r32 and d32
+// (FP32), m per column, and the per-column norms, flags, counts and
+// residuals; growing it frees the previous scratch.
+static rb_status ensure_ir(rb_plan *p, int ncol) {
+    if (ncol <= p->kir) return RB_OK;
+    const size_t n = (size_t)ncol * std::max(1, p->m);
+    void *r = nullptr, *r32 = nullptr, *d32 = nullptr, *meta = nullptr;
+    const size_t mb = (sizeof(unsigned long long) * 2 + sizeof(int) * 2 +
+                       sizeof(double)) * (size_t)ncol;
+    if (hipMalloc(&r, sizeof(double) * n) != hipSuccess ||
+        hipMalloc(&r32, sizeof(float) * n) != hipSuccess ||
+        hipMalloc(&d32, sizeof(float) * n) != hipSuccess ||
+        hipMalloc(&meta, mb) != hipSuccess) {
+        (void)hipGetLastError();
+        for (void *q : {r, r32, d32, meta})
+            if (q) (void)hipFree(q);
+        set_error(RB_ERR_NO_MEMORY, "hipMalloc failed (refinement scratch)");
+        return RB_ERR_NO_MEMORY;
+    }
+    free_ir(p);
+    p->d_r = (double *)r;
+    p->d_r32 = (float *)r32;
+    p->d_d32 = (float *)d32;
+    p->d_norms = (unsigned long long *)meta;
+    p->d_irrel = (double *)(p->d_norms + 2 * (size_t)ncol);
+    p->d_irdone = (int *)(p->d_irrel + ncol);
+    p->d_iriters = p->d_irdone + ncol;
+    p->kir = ncol;
+    return RB_OK;
+}
+
+void free_ir(rb_plan *p) {
+    if (p->kir == 0) return;
+    (void)hipFree(p->d_r);
+    (void)hipFree(p->d_r32);
+    (void)hipFree(p->d_d32);
+    (void)hipFree(p->d_norms);
+    p->d_r = nullptr;
+    p->d_r32 = p->d_d32 = nullptr;
+    p->d_norms = nullptr;
+    p->kir = 0;
+}
+
+// FP32 solves refined in FP64 (rb_solve_f32ir), all right-hand sides of a
+// walk at once: x = 0, r = b; repeat { d = solve32((float) r); x += d;
+// r = b - A x (FP64) } per column until max|r| <= tol * max|b| or
+// max_iter corrections.  The loop runs on the device; the host reads the
+// columns' flags after the second correction and each later one (one
+// synchronisation for the usual two corrections), or never when the
+// caller asks for no report (iters and relres both null): then max_iter
+// corrections run (columns that are done skip their passes) and a column
+// that missed tol sets the plan's status (rb_plan_check).
 rb_status solve_f32ir(rb_plan *p, const double *d_b, int ldb, double *d_x,
                       int ldx, int nrhs, double tol, int max_iter,
                       hipStream_t s, int *iters, double *relres) {
     const int m = p->m;
+    const bool report = iters || relres;
+    const int cap = walk_cols(p);
+    RB_TRY(ensure_ir(p, cap));
     int it_max = 0;
     double rel_max = 0.0;
     bool bad = false;
-    for (int k = 0; k < nrhs && m > 0; k++) {
-        const double *b = d_b + (size_t)k * ldb;
-        double *x = d_x + (size_t)k * ldx;
-        unsigned long long *nrm = p->d_norms;
-        RB_HIP(hipMemsetAsync(nrm, 0, 2 * sizeof(unsigned long long), s));
-        k_absmax<<<std::min(nblk(m), 1024), kT, 0, s>>>(b, m, nrm);
-        RB_HIP(hipMemsetAsync(x, 0, sizeof(double) * (size_t)m, s));
-        RB_HIP(hipMemcpyAsync(p->d_r, b, sizeof(double) * (size_t)m,
-                              hipMemcpyDeviceToDevice, s));
-        unsigned long long hb = 0;
-        RB_HIP(hipMemcpyAsync(&hb, nrm, sizeof(hb), hipMemcpyDeviceToHost, s));
-        RB_HIP(hipStreamSynchronize(s));
-        const double bn = __builtin_bit_cast(double, hb);
-        double rel = 0.0;
-        int it = 0;
-        while (bn > 0.0 && it < max_iter) {
-            it++;
-            k_to_f32<<<nblk(m), kT, 0, s>>>(p->d_r, m, p->d_r32);
-            RB_TRY(walk<float>(p, &p->graph_exec32, p->d_val32, p->d_invd32,
-                               p->d_bperm32, p->d_xperm32, p->d_r32, p->d_d32,
-                               s));
-            k_add_f32<<<nblk(m), kT, 0, s>>>(p->d_d32, m, x);
-            RB_HIP(hipMemsetAsync(nrm + 1, 0, sizeof(unsigned long long), s));
-            k_residual<<<nblk(m), kT, 0, s>>>(p->d_rptr, p->d_rcol, p->d_rval,
-                                              p->d_levelItem, m, b, x, p->d_r,
-                                              nrm + 1);
+    std::vector<int> hdone(cap), hit(cap);
+    std::vector<double> hrel(cap);
+    for (int k0 = 0; k0 < nrhs && m > 0; k0 += cap) {
+        const int nc = std::min(cap, nrhs - k0);
+        const double *b = d_b + (size_t)k0 * ldb;
+        double *x = d_x + (size_t)k0 * ldx;
+        const dim3 g2(nblk(m), nc);
+        k_ir_reset<<<nblk(nc), kT, 0, s>>>(p->d_norms, p->d_irdone,
+                                           p->d_iriters, p->d_irrel, nc);
+        k_ir_init<<<dim3(std::min(nblk(m), 1024), nc), kT, 0, s>>>(
+            b, ldb, x, ldx, p->d_r, m, p->d_norms);
+        // max |b| / max |r| of column c: d_norms[c] / d_norms[nc + c]
+        for (int it = 1; it <= max_iter; it++) {
+            k_ir_to32<<<g2, kT, 0, s>>>(p->d_r, m, p->d_r32, p->d_irdone);
+            RB_TRY(walk<float>(p, nc, p->d_r32, m, p->d_d32, m, s));
+            k_ir_update<<<g2, kT, 0, s>>>(p->d_d32, m, x, ldx, p->d_irdone);
+            k_ir_residual<<<g2, kT, 0, s>>>(p->d_rptr, p->d_rcol, p->d_rval,
+                                            p->d_levelItem, m, b, ldb, x, ldx,
+                                            p->d_r, p->d_norms + nc,
+                                            p->d_irdone);
+            k_ir_check<<<nblk(nc), kT, 0, s>>>(p->d_norms, p->d_norms + nc, nc,
+                                               tol, p->d_irdone, p->d_iriters,
+                                               p->d_irrel, it);
             RB_HIP(hipGetLastError());
-            unsigned long long hr = 0;
-            RB_HIP(hipMemcpyAsync(&hr, nrm + 1, sizeof(hr),
+            if (report && (it >= 2 || it == max_iter)) {
+                RB_HIP(hipMemcpyAsync(hdone.data(), p->d_irdone, sizeof(int) * nc,
+                                      hipMemcpyDeviceToHost, s));
+                RB_HIP(hipStreamSynchronize(s));
+                bool all = true;
+                for (int c = 0; c < nc; c++) all = all && hdone[c];
+                if (all) break;
+            }
+        }
+        if (report) {
+            RB_HIP(hipMemcpyAsync(hdone.data(), p->d_irdone, sizeof(int) * nc,
+                                  hipMemcpyDeviceToHost, s));
+            RB_HIP(hipMemcpyAsync(hit.data(), p->d_iriters, sizeof(int) * nc,
+                                  hipMemcpyDeviceToHost, s));
+            RB_HIP(hipMemcpyAsync(hrel.data(), p->d_irrel, sizeof(double) * nc,
                                   hipMemcpyDeviceToHost, s));
             RB_HIP(hipStreamSynchronize(s));
-            rel = __builtin_bit_cast(double, hr) / bn;
-            if (rel <= tol) break;
+            for (int c = 0; c < nc; c++) {
+                if (!hdone[c]) bad = true;
+                it_max = std::max(it_max, hit[c]);
+                if (hrel[c] != hrel[c] || hrel[c] > rel_max) rel_max = hrel[c];
+            }
+        } else {
+            k_ir_flag<<<nblk(nc), kT, 0, s>>>(p->d_irdone, nc, p->d_status);
+            RB_HIP(hipGetLastError());
         }
-        if (!(rel <= tol)) bad = true;
-        it_max = std::max(it_max, it);
-        if (rel != rel || rel > rel_max) rel_max = rel;
     }
     if (iters) *iters = it_max;
     if (relres) *relres = rel_max;

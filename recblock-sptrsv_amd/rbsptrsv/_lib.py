@@ -60,7 +60,8 @@ class rb_plan_info(ctypes.Structure):
                 ("nblocks", ctypes.c_int), ("nlevel", ctypes.c_int),
                 ("schedule", ctypes.c_int), ("preprocess_ms", ctypes.c_double),
                 ("device_bytes", ctypes.c_size_t),
-                ("phase_ms", ctypes.c_double * 4)]
+                ("phase_ms", ctypes.c_double * 4),
+                ("tune_ms", ctypes.c_double * 2)]
 
 
 def build(jobs=8):

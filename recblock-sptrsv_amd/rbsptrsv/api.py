@@ -167,10 +167,12 @@ class Plan:
                              _stream_ptr(stream)))
         return x
 
-    def solve_f32ir(self, b, x=None, tol=1e-12, max_iter=10, stream=None):
+    def solve_f32ir(self, b, x=None, tol=1e-12, max_iter=10, stream=None,
+                    report=True):
         """FP32 solves with FP64 refinement (plan built with mixed=True).
         b and x as in solve().  Returns (x, iterations, relative
-        residual)."""
+        residual).  report=False: no host synchronisation; returns (x, None,
+        None) and a column that missed tol shows in check()."""
         torch = _torch()
         b, ldb, nrhs = _rhs(torch, b, self.m)
         x, ldx = _out(torch, x, self.m, nrhs, b.dim() == 2)
@@ -178,9 +180,10 @@ class Plan:
         rr = ctypes.c_double(0.0)
         check(lib().rb_solve_f32ir(self._h, _ptr(b), ldb, _ptr(x), ldx, nrhs,
                                    ctypes.c_double(tol), max_iter,
-                                   _stream_ptr(stream), ctypes.byref(it),
-                                   ctypes.byref(rr)))
-        return x, it.value, rr.value
+                                   _stream_ptr(stream),
+                                   ctypes.byref(it) if report else None,
+                                   ctypes.byref(rr) if report else None))
+        return (x, it.value, rr.value) if report else (x, None, None)
 
     def set_timing(self, enable=True):
         """Record events around each solve's main launch (see

@@ -42,33 +42,49 @@ class Server:
     ``Plan.from_factor``).  ``mode``: "fp64" (plain solves) or "f32ir" (FP32
     solves refined in FP64 to ``tol``).  ``mem_budget``: device bytes for
     request buffers in flight.  ``streams``: streams the factors are spread
-    over (default priority)."""
+    over (default priority).  ``log``: called with a progress line after
+    each plan is built and warmed up."""
 
     def __init__(self, factors, mode="fp64", tol=1e-12, max_batch=64,
-                 mem_budget=1 << 30, streams=4, lv=-1, tune_rhs=8):
+                 mem_budget=1 << 30, streams=4, lv=-1, tune_rhs=8, log=None):
         import torch
         if mode not in ("fp64", "f32ir"):
             raise ValueError("mode is fp64 or f32ir")
         self.torch = torch
         self.mode, self.tol = mode, tol
         self.max_batch, self.mem_budget = max_batch, mem_budget
-        self.plans = [Plan.from_factor(f, lv=lv, mixed=(mode == "f32ir"),
-                                       tune_rhs=tune_rhs) for f in factors]
+        self.log = log
+        self.plans = []
+        for j, f in enumerate(factors):
+            t0 = time.perf_counter()
+            self.plans.append(Plan.from_factor(f, lv=lv, tune_rhs=tune_rhs,
+                                               mixed=(mode == "f32ir")))
+            if log:
+                log("plan %d/%d built (n %d, %.2f s)" %
+                    (j + 1, len(factors), self.plans[-1].m,
+                     time.perf_counter() - t0))
         self.n = [p.m for p in self.plans]
         self.streams = [torch.cuda.Stream() for _ in range(max(1, streams))]
         self.warmup()
 
     def warmup(self):
-        """One full-width batch per factor, so that the first requests do
-        not pay for one-time work (multi-column buffers, graph capture)."""
+        """Batches of every width up to max_batch (powers of two) per
+        factor, so that the first requests do not pay for one-time work
+        (multi-column buffers, graph capture)."""
         torch = self.torch
         for f in range(len(self.plans)):
+            t0 = time.perf_counter()
             z = [torch.zeros(self.n[f], dtype=torch.float64, device="cuda")
                  for _ in range(self.max_batch)]
             reqs = [Request(-1, f, b, 0.0) for b in z]
-            self._launch(f, reqs)
-            self._launch(f, reqs[:1])
-        torch.cuda.synchronize()
+            k = self.max_batch
+            while k >= 1:
+                self._launch(f, reqs[:k])
+                k //= 2
+            torch.cuda.synchronize()
+            if self.log:
+                self.log("factor %d warmed up (%.2f s)" %
+                         (f, time.perf_counter() - t0))
 
     def close(self):
         for p in self.plans:
@@ -88,7 +104,10 @@ class Server:
             if self.mode == "fp64":
                 self.plans[f].solve(B, X, stream=st)
             else:
-                self.plans[f].solve_f32ir(B, X, tol=self.tol, stream=st)
+                # no report: the loop runs on the device without host
+                # synchronisation; a column short of tol shows in check()
+                self.plans[f].solve_f32ir(B, X, tol=self.tol, stream=st,
+                                          report=False)
             ev = torch.cuda.Event(enable_timing=True)
             ev.record(st)
         for j, r in enumerate(reqs):

@@ -118,3 +118,35 @@ def test_group_kernel_refused_beyond_ring(torch):
     nz = tri[(bl["method"][tri] != 0) & (bl["blk_m"][tri] > 0)]
     assert (bk["kern"][nz] != GROUP).any()
     assert residual(f, x, b) < RES_TOL
+
+
+def test_group_kernel_repeated_solves_stay_clear(torch):
+    """A wave of the one-workgroup kernel may run ahead of the others only
+    so far that the ring slots it overwrites have no reader left: twenty
+    solves of the 1341^2 Laplacian lower factor (1.8M rows, triangular
+    blocks of ~1300-row levels, every one on the kernel) leave the
+    watchdog clear and give the same x each time.  (Before the round gate,
+    waves ran ahead, overwrote slots other rows still waited for, and those
+    rows waited out the watchdog: the same factor's plan build took
+    111 s.)"""
+    from rbsptrsv import Plan
+    f = gen.laplacian(1341)
+    b = gen.rhs(f.n, seed=13)
+    p = Plan.from_factor(f, lv=-1, schedule=1, block_kernels=3)
+    try:
+        bk, bl = p.block_kernels(), p.blocks()
+        tri = np.arange(0, len(bl["method"]), 2)
+        nz = tri[(bl["method"][tri] != 0) & (bl["blk_m"][tri] > 0)]
+        assert len(nz) > 0 and (bk["kern"][nz] == GROUP).all()
+        db = torch.as_tensor(b, device="cuda")
+        x0 = None
+        for k in range(20):
+            x = p.solve(db)
+            p.check()
+            if x0 is None:
+                x0 = x.clone()
+            else:
+                assert torch.equal(x, x0), k
+        assert residual(f, x0.cpu().numpy(), b) < RES_TOL
+    finally:
+        p.close()

@@ -505,6 +505,19 @@ def test_f32_refinement_reaches_fp64_accuracy(oracle, torch, name, make):
     with pytest.raises(RbError) as ei:
         p.solve_f32ir(torch.as_tensor(b, device="cuda"), tol=1e-14, max_iter=1)
     assert ei.value.status == 9                      # not converged
+    # without a report: the same x, no synchronisation, and a miss shows in
+    # the plan's check (then cleared)
+    xa, it_a, rel_a = p.solve_f32ir(torch.as_tensor(b, device="cuda"),
+                                    tol=1e-12, report=False)
+    assert it_a is None and rel_a is None
+    p.check()
+    assert np.array_equal(xa.cpu().numpy(), x)
+    p.solve_f32ir(torch.as_tensor(b, device="cuda"), tol=1e-14, max_iter=1,
+                  report=False)
+    with pytest.raises(RbError) as ei:
+        p.check()
+    assert ei.value.status == 9
+    p.check()                                        # cleared
 
 
 def test_f32_refinement_many_rhs_and_errors(torch):
