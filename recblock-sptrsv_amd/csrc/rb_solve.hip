@@ -256,7 +256,10 @@ k_tri_syncfree(const int *__restrict__ rowptr, const int *__restrict__ col,
 // 1024 positions ahead -- took ~1.1 us per level on the banded factor
 // against this form's ~0.9 us: a wave waits for its newest loads as a
 // whole, so the reads ahead for some lanes held the others at every
-// level.)
+// level.  Reading a chunk's tags together and its values together after
+// one acquire fence, instead of a tag and a value per entry, was slower
+// too: 1.47 against 0.91 ms on the banded 100k factor, 5.6 against 4.8 ms
+// on the 1341^2 Laplacian.)
 template <class T>
 __global__ void __launch_bounds__(kGroupThreads)
 k_tri_group(const int *__restrict__ rowptr, const int *__restrict__ col,
@@ -758,11 +761,11 @@ void destroy_graph(rb_plan *p) {
     drop_graphs(p, 1);
 }
 
-// Right-hand sides one walk takes: up to 32, within about 1 GiB of
-// permuted b and x.
+// Right-hand sides one walk takes: up to 32, within 4 GiB of permuted b
+// and x (32 columns up to 8M rows).
 static int walk_cols(const rb_plan *p) {
     const long per = 16L * std::max(1, p->m);
-    return (int)std::max(1L, std::min(32L, (1L << 30) / per));
+    return (int)std::max(1L, std::min(32L, (4L << 30) / per));
 }
 
 // Grid width of a walk of ncol columns: the next power of two, at most
