@@ -248,7 +248,9 @@ def serve_factors():
 def bench_serve(args):
     """Poisson request stream over 16 resident factors (Zipf(1.1) choice),
     served with continuous cross-request batching; the same stream served
-    one request per solve for comparison."""
+    one request per solve for comparison, and FP32 solves refined to 1e-12.
+    Each server also serves streams at the other offered rates
+    (--serve-rates): throughput, p50 and p99 per rate."""
     import numpy as np
     from rbsptrsv.serve import Server, poisson_requests
 
@@ -258,8 +260,16 @@ def bench_serve(args):
     facs = serve_factors()
     say("factors", [f.n for f in facs])
     sizes = [f.n for f in facs]
-    reqs = poisson_requests(16, sizes, args.requests, rate=args.rate, seed=9)
-    res = {}
+    # the headline rate first, then the other offered rates of the
+    # latency-vs-load curve (a stream of at most ~4 s at each)
+    rates = [args.rate] + [float(r) for r in args.serve_rates.split(",")
+                           if r and float(r) != args.rate]
+    streams = {}
+    for rate in rates:
+        count = args.requests if rate == args.rate else \
+            max(200, min(args.requests, int(4 * rate)))
+        streams[rate] = poisson_requests(16, sizes, count, rate=rate, seed=9)
+    res, curve = {}, {}
     modes = (("batched_fp64", "fp64", 64), ("unbatched_fp64", "fp64", 1),
              ("batched_f32ir", "f32ir", 64))
     if args.serve_modes:
@@ -271,24 +281,33 @@ def bench_serve(args):
                      log=lambda msg, name=name: say(name, msg))
         build_s = time.perf_counter() - t0
         say(name, "server ready in %.1f s" % build_s)
-        stats, out = srv.run(reqs)
-        for p in srv.plans:
-            p.check()
-        rels = []
-        for r in out[:: max(1, len(out) // 200)]:
-            f = facs[r.factor]
-            b = np.asarray(reqs[r.rid][1])
-            x = r.x.cpu().numpy()
-            rels.append(float(np.linalg.norm(b - f.matvec(x)) /
-                              np.linalg.norm(b)))
-        rels = np.array(rels)
-        stats["residual_sampled_max"] = float(rels.max())   # NaN if any is
-        stats["residual_sampled_nonfinite"] = int((~np.isfinite(rels)).sum())
-        stats["server_build_s"] = round(build_s, 1)
-        res[name] = {k: (round(v, 4) if isinstance(v, float) and
+        curve[name] = []
+        for rate in rates:
+            reqs = streams[rate]
+            stats, out = srv.run(reqs)
+            for p in srv.plans:
+                p.check()
+            rels = []
+            for r in out[:: max(1, len(out) // 200)]:
+                f = facs[r.factor]
+                b = np.asarray(reqs[r.rid][1])
+                x = r.x.cpu().numpy()
+                rels.append(float(np.linalg.norm(b - f.matvec(x)) /
+                                  np.linalg.norm(b)))
+            rels = np.array(rels)
+            stats["residual_sampled_max"] = float(rels.max())  # NaN if any is
+            stats["residual_sampled_nonfinite"] = int((~np.isfinite(rels)).sum())
+            stats["offered_rate_per_s"] = rate
+            stats = {k: (round(v, 4) if isinstance(v, float) and
                          not k.startswith("residual") else v)
                      for k, v in stats.items()}
-        say(name, res[name])
+            say(name, stats)
+            curve[name].append({k: stats[k] for k in (
+                "offered_rate_per_s", "requests", "solves_per_s", "p50_ms",
+                "p99_ms", "mean_batch")})
+            if rate == args.rate:
+                stats["server_build_s"] = round(build_s, 1)
+                res[name] = stats
         srv.close()
     b = res.get("batched_fp64") or next(iter(res.values()))
     out = {
@@ -304,6 +323,7 @@ def bench_serve(args):
                    "requests": args.requests, "zipf": 1.1},
         "p50_ms": b["p50_ms"], "p99_ms": b["p99_ms"],
         "servers": res,
+        "latency_vs_load": curve,
     }
     print(json.dumps(out), flush=True)
 
@@ -330,6 +350,9 @@ def main():
                     help="serve workload: requests in the stream")
     ap.add_argument("--serve-modes", default="",
                     help="serve workload: comma list of servers to run")
+    ap.add_argument("--serve-rates", default="250,1000,2000",
+                    help="serve workload: further offered rates (comma "
+                         "list) for the latency-vs-load curve")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only (this build has no multi-GPU "

@@ -189,6 +189,7 @@ struct rb_plan {
     unsigned long long *d_norms = nullptr;           // max |b|, max |r|
     double *d_irrel = nullptr;                       // relative residuals
     int *d_irdone = nullptr, *d_iriters = nullptr;   // converged, corrections
+    int *d_irleft = nullptr;                         // columns not converged
 };
 
 namespace rb {

@@ -76,12 +76,15 @@ template <class T>
 __global__ void k_prologue(const T *__restrict__ b, long ldb,
                            const int *__restrict__ lf, int m, long ld,
                            T *bperm, T *xperm, int *tickets, int ntri,
-                           int *ncolw) {
+                           int *ncolw, const int *__restrict__ left) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int c = blockIdx.y;
     // the walk's column count, read by its kernels (graphs are captured
-    // for a few grid widths, each solving the first *ncolw columns)
-    if (ncolw && i == 0 && c == 0) *ncolw = gridDim.y;
+    // for a few grid widths, each solving the first *ncolw columns); zero
+    // when a refinement has no column left to correct (*left == 0)
+    const bool skip = left && *left == 0;
+    if (ncolw && i == 0 && c == 0) *ncolw = skip ? 0 : gridDim.y;
+    if (skip) return;
     if (i < m) {
         bperm[c * ld + i] = b[c * ldb + lf[i]];
         ((typename Bits<T>::U *)xperm)[c * ld + i] = Bits<T>::kPend;
@@ -94,7 +97,8 @@ __global__ void k_prologue(const T *__restrict__ b, long ldb,
 template <class T>
 __global__ void k_epilogue(const T *__restrict__ xperm, long ld,
                            const int *__restrict__ lf, int m, T *x, long ldx,
-                           const int *status) {
+                           const int *status, const int *__restrict__ left) {
+    if (left && *left == 0) return;  // the walk was skipped
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int c = blockIdx.y;
     if (i < m)
@@ -567,7 +571,8 @@ __global__ void k_ir_residual(const int *__restrict__ rptr,
 // for the next residual
 __global__ void k_ir_check(const unsigned long long *__restrict__ bmax,
                            unsigned long long *rmax, int ncol, double tol,
-                           int *done, int *iters, double *rel, int it) {
+                           int *done, int *iters, double *rel, int it,
+                           int *left) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncol || done[c]) return;
     const double bn = __longlong_as_double((long long)bmax[c]);
@@ -575,13 +580,17 @@ __global__ void k_ir_check(const unsigned long long *__restrict__ bmax,
     const double q = bn > 0.0 ? rn / bn : 0.0;
     rel[c] = q;
     iters[c] = it;
-    if (q <= tol) done[c] = 1;
+    if (q <= tol) {
+        done[c] = 1;
+        atomicSub(left, 1);
+    }
     rmax[c] = 0;
 }
 
 __global__ void k_ir_reset(unsigned long long *norms, int *done, int *iters,
-                           double *rel, int ncol) {
+                           double *rel, int ncol, int *left) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c == 0) *left = ncol;
     if (c < ncol) {
         norms[c] = 0;
         norms[ncol + c] = 0;
@@ -819,7 +828,7 @@ static rb_status ensure_cols(rb_plan *p, int ncol, hipStream_t s) {
 
 template <class T>
 static rb_status walk(rb_plan *p, int ncol, const T *b, long ldb, T *x,
-                      long ldx, hipStream_t s) {
+                      long ldx, hipStream_t s, const int *left = nullptr) {
     constexpr int ty = sizeof(T) == 8 ? 0 : 1;
     const int m = p->m;
     const int width = walk_width(p, ncol);
@@ -833,7 +842,7 @@ static rb_status walk(rb_plan *p, int ncol, const T *b, long ldb, T *x,
     const int pro = nblk(m > p->ntri ? m : p->ntri);
     k_prologue<T><<<dim3(pro, ncol), kT, 0, s>>>(b, ldb, p->d_levelItem, m, m,
                                                  bperm, xperm, p->d_tickets,
-                                                 p->ntri, p->d_ncol);
+                                                 p->ntri, p->d_ncol, left);
     mark_start(p, s);
     if (graph)
         RB_HIP(hipGraphLaunch((hipGraphExec_t)exec, s));
@@ -842,7 +851,7 @@ static rb_status walk(rb_plan *p, int ncol, const T *b, long ldb, T *x,
                          p->d_ncol, s);
     mark_end(p, s);
     k_epilogue<T><<<dim3(nblk(m), ncol), kT, 0, s>>>(xperm, m, p->d_levelItem,
-                                                     m, x, ldx, p->d_status);
+                                                     m, x, ldx, p->d_status, left);
     RB_HIP(hipGetLastError());
     return RB_OK;
 }
@@ -877,14 +886,15 @@ rb_status solve_blocked_ws(rb_plan *p, const double *d_b, int ldb,
         const int pro = nblk(m > p->ntri ? m : p->ntri);
         k_prologue<double><<<pro, kT, 0, s>>>(d_b + (size_t)k * ldb, ldb,
                                               p->d_levelItem, m, m, bperm,
-                                              xperm, tickets, p->ntri, nullptr);
+                                              xperm, tickets, p->ntri, nullptr,
+                                              nullptr);
         mark_start(p, s);
         launch_blocks<double>(p, p->d_val, p->d_invd, bperm, xperm, tickets,
                               1, m, p->d_ncol + 1, s);
         mark_end(p, s);
         k_epilogue<double><<<nblk(m), kT, 0, s>>>(
             xperm, m, p->d_levelItem, m, d_x + (size_t)k * ldx, ldx,
-            p->d_status);
+            p->d_status, nullptr);
         RB_HIP(hipGetLastError());
     }
     return RB_OK;
@@ -1051,7 +1061,7 @@ static rb_status ensure_ir(rb_plan *p, int ncol) {
     const size_t n = (size_t)ncol * std::max(1, p->m);
     void *r = nullptr, *r32 = nullptr, *d32 = nullptr, *meta = nullptr;
     const size_t mb = (sizeof(unsigned long long) * 2 + sizeof(int) * 2 +
-                       sizeof(double)) * (size_t)ncol;
+                       sizeof(double)) * (size_t)ncol + sizeof(int);
     if (hipMalloc(&r, sizeof(double) * n) != hipSuccess ||
         hipMalloc(&r32, sizeof(float) * n) != hipSuccess ||
         hipMalloc(&d32, sizeof(float) * n) != hipSuccess ||
@@ -1070,6 +1080,7 @@ static rb_status ensure_ir(rb_plan *p, int ncol) {
     p->d_irrel = (double *)(p->d_norms + 2 * (size_t)ncol);
     p->d_irdone = (int *)(p->d_irrel + ncol);
     p->d_iriters = p->d_irdone + ncol;
+    p->d_irleft = p->d_iriters + ncol;
     p->kir = ncol;
     return RB_OK;
 }
@@ -1113,13 +1124,16 @@ rb_status solve_f32ir(rb_plan *p, const double *d_b, int ldb, double *d_x,
         double *x = d_x + (size_t)k0 * ldx;
         const dim3 g2(nblk(m), nc);
         k_ir_reset<<<nblk(nc), kT, 0, s>>>(p->d_norms, p->d_irdone,
-                                           p->d_iriters, p->d_irrel, nc);
+                                           p->d_iriters, p->d_irrel, nc,
+                                           p->d_irleft);
         k_ir_init<<<dim3(std::min(nblk(m), 1024), nc), kT, 0, s>>>(
             b, ldb, x, ldx, p->d_r, m, p->d_norms);
         // max |b| / max |r| of column c: d_norms[c] / d_norms[nc + c]
         for (int it = 1; it <= max_iter; it++) {
             k_ir_to32<<<g2, kT, 0, s>>>(p->d_r, m, p->d_r32, p->d_irdone);
-            RB_TRY(walk<float>(p, nc, p->d_r32, m, p->d_d32, m, s));
+            // once every column has converged the walk is skipped (its
+            // prologue sets the width its kernels read to zero)
+            RB_TRY(walk<float>(p, nc, p->d_r32, m, p->d_d32, m, s, p->d_irleft));
             k_ir_update<<<g2, kT, 0, s>>>(p->d_d32, m, x, ldx, p->d_irdone);
             k_ir_residual<<<g2, kT, 0, s>>>(p->d_rptr, p->d_rcol, p->d_rval,
                                             p->d_levelItem, m, b, ldb, x, ldx,
@@ -1127,7 +1141,7 @@ rb_status solve_f32ir(rb_plan *p, const double *d_b, int ldb, double *d_x,
                                             p->d_irdone);
             k_ir_check<<<nblk(nc), kT, 0, s>>>(p->d_norms, p->d_norms + nc, nc,
                                                tol, p->d_irdone, p->d_iriters,
-                                               p->d_irrel, it);
+                                               p->d_irrel, it, p->d_irleft);
             RB_HIP(hipGetLastError());
             if (report && (it >= 2 || it == max_iter)) {
                 RB_HIP(hipMemcpyAsync(hdone.data(), p->d_irdone, sizeof(int) * nc,
