@@ -458,7 +458,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(kern_gbs, 3),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(kern_gbs / HBM_PEAK_GBS, 5),
-                     "traffic": measured_traffic(wname, sched),
+                     # counter runs cover one right-hand side
+                     "traffic": measured_traffic(wname, sched)
+                                if args.rhs == 1 else None,
                      "kernel": "k_fused" if sched == "fused" else
                                "blocked block walk (graph of block kernels)",
                      "launch_ms": round(launch_ms, 5),

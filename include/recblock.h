@@ -317,11 +317,15 @@ rb_plan *rb_batch_plan(rb_batch *batch);
 rb_status rb_batch_destroy(rb_batch *batch);
 
 /* Kernel timing for measurement harnesses.  While enabled, every solve
- * records HIP events on its stream around its main launch: the fused
+ * records HIP events on its stream around its main launches: the fused
  * kernel (fused schedule) or the captured block walk (blocked schedule),
- * without the passes that move b and x between orders.
- * rb_plan_kernel_ms waits for the last such launch and returns its
- * duration in milliseconds.  No counterpart in the reference. */
+ * without the passes that move b and x between orders, from the start of
+ * the solve's first main launch to the end of its last (a solve of more
+ * columns than one launch or walk takes makes several; the passes between
+ * them are then inside the span; for rb_solve_f32ir the span covers all of
+ * its walks).  rb_plan_kernel_ms waits for the last solve's span and
+ * returns its duration in milliseconds.  No counterpart in the
+ * reference. */
 rb_status rb_plan_set_timing(rb_plan *plan, int enable);
 rb_status rb_plan_kernel_ms(rb_plan *plan, double *ms);
 

@@ -235,6 +235,7 @@ rb_status rb_solve(rb_plan *p, const double *d_b, int ldb, double *d_x,
         return RB_ERR_INVALID_ARG;
     }
     if (nrhs == 0) return ok();
+    p->mark_first = true;
     rb_status st = p->schedule == RB_SCHED_FUSED
                        ? solve_fused(p, d_b, ldb, d_x, ldx, nrhs,
                                      (hipStream_t)stream)
@@ -288,6 +289,7 @@ rb_status rb_solve_f32ir(rb_plan *p, const double *d_b, int ldb, double *d_x,
     if (iters) *iters = 0;
     if (relres) *relres = 0.0;
     if (nrhs == 0) return ok();
+    p->mark_first = true;
     rb_status st = solve_f32ir(p, d_b, ldb, d_x, ldx, nrhs, tol, max_iter,
                                (hipStream_t)stream, iters, relres);
     if (st != RB_OK) return st;

@@ -174,6 +174,7 @@ struct rb_plan {
     int kcols = 1, kcols32 = 1, kticket = 1;
     // kernel timing (rb_plan_set_timing)
     bool timing = false;
+    bool mark_first = false;  // the next main launch opens the timed span
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
     // FP32 solve + FP64 refinement (rb_opts.mixed)
@@ -227,9 +228,11 @@ rb_status solve_blocked_ws(rb_plan *p, const double *d_b, int ldb,
 rb_status solve_fused_ws(rb_plan *p, const double *d_b, int ldb, double *d_x,
                          int ldx, int nrhs, hipStream_t s, double *ws,
                          size_t ws_bytes);
-// events around a solve's main launch when timing is on
+// events around a solve's main launches when timing is on: from the start
+// of the first (the entry point sets mark_first) to the end of the last
 inline void mark_start(rb_plan *p, hipStream_t s) {
-    if (p->timing) (void)hipEventRecord(p->ev0, s);
+    if (p->timing && p->mark_first) (void)hipEventRecord(p->ev0, s);
+    p->mark_first = false;
 }
 inline void mark_end(rb_plan *p, hipStream_t s) {
     if (p->timing) (void)hipEventRecord(p->ev1, s);

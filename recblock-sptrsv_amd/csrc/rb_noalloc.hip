@@ -90,6 +90,7 @@ extern "C" rb_status rb_solve_noalloc(rb_plan *p, const double *d_b, int ldb,
     void *ws = nullptr;
     RB_TRY(scratch_ptr(dev, &ws));
     const hipStream_t s = (hipStream_t)stream;
+    p->mark_first = true;
     rb_status st =
         fused ? solve_fused_ws(p, d_b, ldb, d_x, ldx, nrhs, s, (double *)ws,
                                kScratchBytes)
