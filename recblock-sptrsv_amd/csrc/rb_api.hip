@@ -236,6 +236,7 @@ rb_status rb_solve(rb_plan *p, const double *d_b, int ldb, double *d_x,
     }
     if (nrhs == 0) return ok();
     p->mark_first = true;
+    rowcap_for_call(p, (hipStream_t)stream);
     rb_status st = p->schedule == RB_SCHED_FUSED
                        ? solve_fused(p, d_b, ldb, d_x, ldx, nrhs,
                                      (hipStream_t)stream)
@@ -290,6 +291,7 @@ rb_status rb_solve_f32ir(rb_plan *p, const double *d_b, int ldb, double *d_x,
     if (relres) *relres = 0.0;
     if (nrhs == 0) return ok();
     p->mark_first = true;
+    rowcap_for_call(p, (hipStream_t)stream);
     rb_status st = solve_f32ir(p, d_b, ldb, d_x, ldx, nrhs, tol, max_iter,
                                (hipStream_t)stream, iters, relres);
     if (st != RB_OK) return st;

@@ -22,6 +22,7 @@ struct rb_batch {
     rb_plan *plan = nullptr;
     rb::Arena arena;
     int *d_fid = nullptr;   // factor of each union row
+    int *d_rowcap = nullptr;  // per permuted row: its factor's column count
     int *d_off = nullptr;   // [nf] row offsets
     // per-call tables (device, filled from pinned host staging)
     const double **d_bp = nullptr;
@@ -75,6 +76,14 @@ __global__ void k_gather(const int *__restrict__ fid, const int *__restrict__ of
     const int f = fid[i];
     const int nr = ld[2 * nf + f];
     B[t] = k < nr ? bp[f][k * (long)ld[f] + (i - off[f])] : 0.0;
+}
+
+// the union plan's row caps for one call: permuted row i (original row
+// lf[i]) solves as many columns as its factor has
+__global__ void k_rowcap(const int *__restrict__ lf, const int *__restrict__ fid,
+                         const int *__restrict__ nrhs, int n, int *rowcap) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) rowcap[i] = nrhs[fid[lf[i]]];
 }
 
 __global__ void k_scatter(const int *__restrict__ fid, const int *__restrict__ off,
@@ -151,6 +160,7 @@ rb_status rb_batch_create(int nf, const int *const *d_colptr,
         (st = tmp.alloc(&uri, nnz_total)) != RB_OK ||
         (st = tmp.alloc(&uv, nnz_total)) != RB_OK ||
         (st = B->arena.alloc(&B->d_fid, n_total)) != RB_OK ||
+        (st = B->arena.alloc(&B->d_rowcap, std::max(1L, n_total))) != RB_OK ||
         (st = B->arena.alloc(&B->d_off, nf)) != RB_OK ||
         (st = B->arena.alloc(&B->d_bp, nf)) != RB_OK ||
         (st = B->arena.alloc(&B->d_xp, nf)) != RB_OK ||
@@ -186,6 +196,11 @@ rb_status rb_batch_create(int nf, const int *const *d_colptr,
     (void)hipStreamSynchronize(s);
     tmp.release();
     if (st != RB_OK) return fail(st);
+    // the blocked walk skips the rows of factors with fewer columns than
+    // the batch's widest (graphs captured while tuning hold no caps)
+    B->plan->d_rowcap = B->d_rowcap;
+    B->plan->rowcap_all = false;
+    destroy_graph(B->plan);
     B->stage_bytes = sizeof(void *) * 2 * nf + sizeof(int) * 3 * nf;
     if (hipHostMalloc(&B->h_stage, B->stage_bytes) != hipSuccess ||
         hipEventCreateWithFlags(&B->staged, hipEventDisableTiming) != hipSuccess) {
@@ -253,7 +268,12 @@ static rb_status batch_stage(rb_batch *B, const double *const *d_b,
     const long tot = (long)B->n_total * k;
     k_gather<<<nblk(tot), kT, 0, s>>>(B->d_fid, B->d_off, B->d_bp, B->d_ld, nf,
                                       B->n_total, k, B->d_B);
+    k_rowcap<<<nblk(B->n_total), kT, 0, s>>>(B->plan->d_levelItem, B->d_fid,
+                                             B->d_ld + 2 * nf, B->n_total,
+                                             B->d_rowcap);
     RB_HIP(hipGetLastError());
+    B->plan->rowcap_set = true;
+    B->plan->rowcap_all = false;
     return RB_OK;
 }
 

@@ -157,6 +157,12 @@ struct rb_plan {
     int *d_tickets = nullptr;   // one per triangular block
     int *d_status = nullptr;    // device watchdog flags (0 = healthy)
     int *d_ncol = nullptr;      // [0] columns of the running walk, [1] = 1
+    // batch plans: per permuted row, the columns its factor has in the
+    // current batch solve (the walk skips the row in later columns); the
+    // entry points other than the batch's reset it to "all" before a solve
+    int *d_rowcap = nullptr;
+    bool rowcap_set = false;    // the batch has filled d_rowcap for this solve
+    bool rowcap_all = false;    // d_rowcap holds "every column" everywhere
     unsigned spin_limit = 1u << 22;  // polls before a bounded wait gives up
     int scratch_rhs = 0;
 
@@ -238,6 +244,9 @@ inline void mark_end(rb_plan *p, hipStream_t s) {
     if (p->timing) (void)hipEventRecord(p->ev1, s);
 }
 void destroy_graph(rb_plan *p);
+// at the start of a solve call: a batch plan's row caps back to "every
+// column" unless the batch has just set them for this call
+void rowcap_for_call(rb_plan *p, hipStream_t s);
 // frees the right-hand-side and refinement scratch grown beyond the plan's
 // own (one column)
 void free_solve_scratch(rb_plan *p);

@@ -91,6 +91,7 @@ extern "C" rb_status rb_solve_noalloc(rb_plan *p, const double *d_b, int ldb,
     RB_TRY(scratch_ptr(dev, &ws));
     const hipStream_t s = (hipStream_t)stream;
     p->mark_first = true;
+    rowcap_for_call(p, s);
     rb_status st =
         fused ? solve_fused_ws(p, d_b, ldb, d_x, ldx, nrhs, s, (double *)ws,
                                kScratchBytes)

@@ -97,24 +97,35 @@ __global__ void k_prologue(const T *__restrict__ b, long ldb,
 template <class T>
 __global__ void k_epilogue(const T *__restrict__ xperm, long ld,
                            const int *__restrict__ lf, int m, T *x, long ldx,
-                           const int *status, const int *__restrict__ left) {
+                           const int *status, const int *__restrict__ left,
+                           const int *__restrict__ rowcap) {
     if (left && *left == 0) return;  // the walk was skipped
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int c = blockIdx.y;
     if (i < m)
         x[c * ldx + lf[i]] = (*status & 7) ? __builtin_bit_cast(T, Bits<T>::kQNaN)
-                                           : xperm[c * ld + i];
+                             : rowcap && c >= rowcap[i] ? T(0)  // not solved
+                                                        : xperm[c * ld + i];
+}
+
+// rowcap (may be null): per permuted row, the columns its factor has in a
+// batch; the walk kernels leave the row alone in later columns (nothing
+// outside the factor depends on it)
+__device__ inline bool skip_row(const int *__restrict__ rowcap, int i) {
+    return rowcap && (int)blockIdx.y >= rowcap[i];
 }
 
 template <class T>
 __global__ void k_tri_diag(const T *__restrict__ invd,
                            const T *__restrict__ bperm, int up, int n,
-                           T *xperm, long ld, const int *__restrict__ ncolp) {
+                           T *xperm, long ld, const int *__restrict__ ncolp,
+                           const int *__restrict__ rowcap) {
     if ((int)blockIdx.y >= *ncolp) return;
     bperm += blockIdx.y * ld;
     xperm += blockIdx.y * ld;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) st_x(&xperm[up + i], bperm[up + i] * invd[up + i]);
+    if (i < n && !skip_row(rowcap, up + i))
+        st_x(&xperm[up + i], bperm[up + i] * invd[up + i]);
 }
 
 // level-set, thread per row; rows of one local level
@@ -125,13 +136,15 @@ __global__ void k_tri_level_thread(const int *__restrict__ rowptr,
                                    const T *__restrict__ invd,
                                    const T *__restrict__ bperm,
                                    const int *__restrict__ rows, int nrows,
-                                   int sb, int up, T *xperm, long ld, const int *__restrict__ ncolp) {
+                                   int sb, int up, T *xperm, long ld, const int *__restrict__ ncolp,
+                                   const int *__restrict__ rowcap) {
     if ((int)blockIdx.y >= *ncolp) return;
     bperm += blockIdx.y * ld;
     xperm += blockIdx.y * ld;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nrows) return;
     const int r = rows[k];
+    if (skip_row(rowcap, up + r)) return;
     const int s = rowptr[sb + r], e = rowptr[sb + r + 1];
     T acc = 0;
     for (int j = s; j < e; j++) acc += val[j] * xperm[up + col[j]];
@@ -159,7 +172,8 @@ __global__ void k_tri_level_wave(const int *__restrict__ rowptr,
                                  const T *__restrict__ invd,
                                  const T *__restrict__ bperm,
                                  const int *__restrict__ rows, int nrows,
-                                 int sb, int up, T *xperm, long ld, const int *__restrict__ ncolp) {
+                                 int sb, int up, T *xperm, long ld, const int *__restrict__ ncolp,
+                                 const int *__restrict__ rowcap) {
     if ((int)blockIdx.y >= *ncolp) return;
     bperm += blockIdx.y * ld;
     xperm += blockIdx.y * ld;
@@ -167,6 +181,7 @@ __global__ void k_tri_level_wave(const int *__restrict__ rowptr,
     const int lane = threadIdx.x & 63;
     if (k >= nrows) return;
     const int r = rows[k];
+    if (skip_row(rowcap, up + r)) return;  // the whole wave: one row
     const int s = rowptr[sb + r], e = rowptr[sb + r + 1];
     T acc = 0;
     for (int j = s + lane; j < e; j += 64) acc += val[j] * xperm[up + col[j]];
@@ -181,7 +196,8 @@ k_tri_syncfree(const int *__restrict__ rowptr, const int *__restrict__ col,
                const T *__restrict__ val, const T *__restrict__ invd,
                const T *__restrict__ bperm, int sb, int up, int bm,
                int subst, int *ticket, T *xperm, int *status,
-               unsigned spin_limit, long ld, int tstride, const int *__restrict__ ncolp) {
+               unsigned spin_limit, long ld, int tstride, const int *__restrict__ ncolp,
+               const int *__restrict__ rowcap) {
     if ((int)blockIdx.y >= *ncolp) return;
     bperm += blockIdx.y * ld;
     xperm += blockIdx.y * ld;
@@ -191,8 +207,8 @@ k_tri_syncfree(const int *__restrict__ rowptr, const int *__restrict__ col,
     if (lane == 0) tile = atomicAdd(ticket, 1);
     tile = __shfl(tile, 0, 64);
     const int pos = tile * 64 + lane;
-    const bool valid = pos < bm;
     const int r = subst == 0 ? pos : bm - 1 - pos;
+    const bool valid = pos < bm && !skip_row(rowcap, up + r);
     int j = 0, e = 0;
     T bv = 0, iv = 0;
     if (valid) {
@@ -269,7 +285,8 @@ __global__ void __launch_bounds__(kGroupThreads)
 k_tri_group(const int *__restrict__ rowptr, const int *__restrict__ col,
             const T *__restrict__ val, const T *__restrict__ invd,
             const T *__restrict__ bperm, int sb, int up, int bm, int subst,
-            T *xperm, int *status, unsigned spin_limit, long ld, const int *__restrict__ ncolp) {
+            T *xperm, int *status, unsigned spin_limit, long ld, const int *__restrict__ ncolp,
+            const int *__restrict__ rowcap) {
     if ((int)blockIdx.y >= *ncolp) return;
     bperm += blockIdx.y * ld;
     xperm += blockIdx.y * ld;
@@ -301,8 +318,10 @@ k_tri_group(const int *__restrict__ rowptr, const int *__restrict__ col,
             }
         }
         const int pos = p0 + (threadIdx.x & 63);
-        const bool valid = pos < bm;
         const int r = subst == 0 ? pos : bm - 1 - pos;
+        // a skipped row publishes nothing: only rows of its own factor,
+        // skipped too, read it
+        const bool valid = pos < bm && !skip_row(rowcap, up + r);
         int jc = 0, e = 0;  // chunk base, row end
         T bv = 0, iv = 0;
         if (valid) {
@@ -399,7 +418,8 @@ __global__ void __launch_bounds__(kT)
 k_sq_group(const int *__restrict__ rowptr, const int *__restrict__ col,
            const T *__restrict__ val, const T *__restrict__ xperm,
            const int *__restrict__ rows, int nrows, int sb, int up, int left,
-           T *bperm, long ld, const int *__restrict__ ncolp) {
+           T *bperm, long ld, const int *__restrict__ ncolp,
+           const int *__restrict__ rowcap) {
     const int c0 = blockIdx.y * KC;
     const int ncol = *ncolp;
     if (c0 >= ncol) return;
@@ -409,7 +429,8 @@ k_sq_group(const int *__restrict__ rowptr, const int *__restrict__ col,
     const int r = DCSR ? rows[k] : k;
     const int s = rowptr[sb + r], e = rowptr[sb + r + 1];
     if (s == e || e - s > kLongRow) return;
-    const int nc = min(KC, ncol - c0);
+    const int nc = min(KC, min(ncol, rowcap ? rowcap[up + r] : ncol) - c0);
+    if (nc <= 0) return;
     const T *x0 = xperm + c0 * ld + left;
     T acc[KC];
 #pragma unroll
@@ -436,11 +457,12 @@ __global__ void __launch_bounds__(kT)
 k_sq_long(const int *__restrict__ rowptr, const int *__restrict__ col,
           const T *__restrict__ val, const T *__restrict__ xperm,
           const int *__restrict__ rows, int sb, int up, int left, T *bperm,
-          long ld, const int *__restrict__ ncolp) {
+          long ld, const int *__restrict__ ncolp, const int *__restrict__ rowcap) {
     if ((int)blockIdx.y >= *ncolp) return;
     xperm += blockIdx.y * ld;
     bperm += blockIdx.y * ld;
     const int r = rows[blockIdx.x];
+    if (skip_row(rowcap, up + r)) return;  // the whole workgroup
     const int s = rowptr[sb + r], e = rowptr[sb + r + 1];
     T acc = 0;
     for (int j = s + threadIdx.x; j < e; j += kT)
@@ -472,10 +494,11 @@ inline int cols_per_thread(int ncol) { return ncol >= 4 ? 4 : ncol >= 2 ? 2 : 1;
 template <class T, bool DCSR, int KC>
 void launch_sq_kc(int g, int nrows, int ncol, hipStream_t s, const int *rowptr,
                   const int *col, const T *val, const T *xperm, const int *rows,
-                  int sb, int up, int left, T *bperm, long ld, const int *ncolp) {
+                  int sb, int up, int left, T *bperm, long ld, const int *ncolp,
+                  const int *rowcap) {
     const long th = (long)nrows * g;
     const dim3 grid(nblk(th), (ncol + KC - 1) / KC);
-#define RB_SQ(G) k_sq_group<T, G, DCSR, KC><<<grid, kT, 0, s>>>(rowptr, col, val, xperm, rows, nrows, sb, up, left, bperm, ld, ncolp)
+#define RB_SQ(G) k_sq_group<T, G, DCSR, KC><<<grid, kT, 0, s>>>(rowptr, col, val, xperm, rows, nrows, sb, up, left, bperm, ld, ncolp, rowcap)
     switch (g) {
     case 1: RB_SQ(1); break;
     case 2: RB_SQ(2); break;
@@ -491,11 +514,12 @@ void launch_sq_kc(int g, int nrows, int ncol, hipStream_t s, const int *rowptr,
 template <class T, bool DCSR>
 void launch_sq(int g, int nrows, int ncol, hipStream_t s, const int *rowptr,
                const int *col, const T *val, const T *xperm, const int *rows,
-               int sb, int up, int left, T *bperm, long ld, const int *ncolp) {
+               int sb, int up, int left, T *bperm, long ld, const int *ncolp,
+               const int *rowcap) {
     switch (cols_per_thread(ncol)) {
-    case 4: launch_sq_kc<T, DCSR, 4>(g, nrows, ncol, s, rowptr, col, val, xperm, rows, sb, up, left, bperm, ld, ncolp); break;
-    case 2: launch_sq_kc<T, DCSR, 2>(g, nrows, ncol, s, rowptr, col, val, xperm, rows, sb, up, left, bperm, ld, ncolp); break;
-    default: launch_sq_kc<T, DCSR, 1>(g, nrows, ncol, s, rowptr, col, val, xperm, rows, sb, up, left, bperm, ld, ncolp); break;
+    case 4: launch_sq_kc<T, DCSR, 4>(g, nrows, ncol, s, rowptr, col, val, xperm, rows, sb, up, left, bperm, ld, ncolp, rowcap); break;
+    case 2: launch_sq_kc<T, DCSR, 2>(g, nrows, ncol, s, rowptr, col, val, xperm, rows, sb, up, left, bperm, ld, ncolp, rowcap); break;
+    default: launch_sq_kc<T, DCSR, 1>(g, nrows, ncol, s, rowptr, col, val, xperm, rows, sb, up, left, bperm, ld, ncolp, rowcap); break;
     }
 }
 
@@ -625,6 +649,11 @@ __global__ void k_tri_maxdist(const int *__restrict__ rowptr,
 
 // before a timed run of a triangular block: its x slots back to the
 // sentinel and its ticket to zero
+__global__ void k_fill_int(int *p, long n, int v) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 __global__ void k_block_reset(double *x, int n, int *ticket) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) ((unsigned long long *)x)[i] = kPending;
@@ -645,11 +674,12 @@ static void launch_block(rb_plan *p, int i, const T *val, const T *invd,
     if (B.kind == 0) {
         if (B.method == 0) {
             k_tri_diag<T><<<dim3(nblk(B.blk_m), ncol), kT, 0, s>>>(
-                invd, bperm, B.up, B.blk_m, xperm, ld, nc);
+                invd, bperm, B.up, B.blk_m, xperm, ld, nc, p->d_rowcap);
         } else if (B.kern == kTriGroup) {
             k_tri_group<T><<<dim3(1, ncol), kGroupThreads, 0, s>>>(
                 p->d_rowptr, p->d_col, val, invd, bperm, B.slot_base, B.up,
-                B.blk_m, p->subst, xperm, p->d_status, p->spin_limit, ld, nc);
+                B.blk_m, p->subst, xperm, p->d_status, p->spin_limit, ld, nc,
+                p->d_rowcap);
         } else if (B.kern == kTriRef && B.method == 2) {
             for (int L = 0; L < B.nlv; L++) {
                 const int a = B.lvptr_h[L], c = B.lvptr_h[L + 1];
@@ -659,17 +689,17 @@ static void launch_block(rb_plan *p, int i, const T *val, const T *invd,
                 if (L == 0 || B.nnzlv_h[L] / nr <= 15)
                     k_tri_level_thread<T><<<dim3(nblk(nr), ncol), kT, 0, s>>>(
                         p->d_rowptr, p->d_col, val, invd, bperm, rows, nr,
-                        B.slot_base, B.up, xperm, ld, nc);
+                        B.slot_base, B.up, xperm, ld, nc, p->d_rowcap);
                 else
                     k_tri_level_wave<T><<<dim3(nblk((long)nr * 64), ncol), kT, 0, s>>>(
                         p->d_rowptr, p->d_col, val, invd, bperm, rows, nr,
-                        B.slot_base, B.up, xperm, ld, nc);
+                        B.slot_base, B.up, xperm, ld, nc, p->d_rowcap);
             }
         } else {  // methods 1 and 3, or kTriSyncfree
             k_tri_syncfree<T><<<dim3(nblk((long)B.blk_m, 64), ncol), 64, 0, s>>>(
                 p->d_rowptr, p->d_col, val, invd, bperm, B.slot_base,
                 B.up, B.blk_m, p->subst, tickets + i / 2, xperm,
-                p->d_status, p->spin_limit, ld, p->ntri, nc);
+                p->d_status, p->spin_limit, ld, p->ntri, nc, p->d_rowcap);
         }
         return;
     }
@@ -682,14 +712,16 @@ static void launch_block(rb_plan *p, int i, const T *val, const T *invd,
     if (B.method < 0 || nr == 0) return;  // empty block
     if (dcsr)
         launch_sq<T, true>(B.lanes, nr, ncol, s, p->d_rowptr, p->d_col, val,
-                           xperm, rows, B.slot_base, B.up, B.left, bperm, ld, nc);
+                           xperm, rows, B.slot_base, B.up, B.left, bperm, ld, nc,
+                           p->d_rowcap);
     else
         launch_sq<T, false>(B.lanes, nr, ncol, s, p->d_rowptr, p->d_col, val,
-                            xperm, rows, B.slot_base, B.up, B.left, bperm, ld, nc);
+                            xperm, rows, B.slot_base, B.up, B.left, bperm, ld, nc,
+                            p->d_rowcap);
     if (B.longrow > 0)
         k_sq_long<T><<<dim3(B.longrow, ncol), kT, 0, s>>>(
             p->d_rowptr, p->d_col, val, xperm, p->d_long_rows + B.long_base,
-            B.slot_base, B.up, B.left, bperm, ld, nc);
+            B.slot_base, B.up, B.left, bperm, ld, nc, p->d_rowcap);
 }
 
 // The block walk of one solve in value type T: kernels on plan-owned
@@ -768,6 +800,18 @@ static void drop_graphs(rb_plan *p, int ty) {
 void destroy_graph(rb_plan *p) {
     drop_graphs(p, 0);
     drop_graphs(p, 1);
+}
+
+void rowcap_for_call(rb_plan *p, hipStream_t s) {
+    if (!p->d_rowcap) return;
+    if (p->rowcap_set) {
+        p->rowcap_set = false;
+        return;
+    }
+    if (!p->rowcap_all && p->m > 0) {
+        k_fill_int<<<nblk(p->m), kT, 0, s>>>(p->d_rowcap, p->m, 0x7fffffff);
+        p->rowcap_all = true;
+    }
 }
 
 // Right-hand sides one walk takes: up to 32, within 4 GiB of permuted b
@@ -851,7 +895,8 @@ static rb_status walk(rb_plan *p, int ncol, const T *b, long ldb, T *x,
                          p->d_ncol, s);
     mark_end(p, s);
     k_epilogue<T><<<dim3(nblk(m), ncol), kT, 0, s>>>(xperm, m, p->d_levelItem,
-                                                     m, x, ldx, p->d_status, left);
+                                                     m, x, ldx, p->d_status, left,
+                                                     p->d_rowcap);
     RB_HIP(hipGetLastError());
     return RB_OK;
 }
@@ -894,7 +939,7 @@ rb_status solve_blocked_ws(rb_plan *p, const double *d_b, int ldb,
         mark_end(p, s);
         k_epilogue<double><<<nblk(m), kT, 0, s>>>(
             xperm, m, p->d_levelItem, m, d_x + (size_t)k * ldx, ldx,
-            p->d_status, nullptr);
+            p->d_status, nullptr, p->d_rowcap);
         RB_HIP(hipGetLastError());
     }
     return RB_OK;

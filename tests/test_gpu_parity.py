@@ -586,6 +586,38 @@ def test_batch_matches_single_factor_solves(oracle, torch, schedule):
             assert torch.equal(x, x2)
 
 
+@pytest.mark.parametrize("schedule", [1, 2])
+def test_batch_plan_solved_directly_after_batches(torch, schedule):
+    """A batch's blocked walk skips, in each column, the rows of factors
+    with fewer columns; a solve called on the union plan itself (not
+    through the batch) afterwards solves every row in every column."""
+    import ctypes
+    from rbsptrsv import Batch
+    from rbsptrsv._lib import check, lib
+    facs, nrhs = _small_batch()
+    bt = Batch(facs, lv=2, schedule=schedule)
+    rng = np.random.default_rng(13)
+    bs = [None if k == 0 else torch.as_tensor(rng.uniform(-1, 1, (k, f.n)),
+                                              device="cuda").t()
+          for f, k in zip(facs, nrhs)]
+    bt.solve(bs)
+    n = sum(f.n for f in facs)
+    k = max(nrhs)
+    Bu = rng.uniform(-1, 1, (k, n))
+    Bd = torch.as_tensor(Bu, device="cuda")
+    Xd = torch.zeros_like(Bd)
+    check(lib().rb_solve(lib().rb_batch_plan(bt._h),
+                         ctypes.c_void_p(Bd.data_ptr()), n,
+                         ctypes.c_void_p(Xd.data_ptr()), n, k, None))
+    X = Xd.cpu().numpy()
+    off = 0
+    for f in facs:
+        for j in range(k):
+            assert residual(f, X[j, off:off + f.n], Bu[j, off:off + f.n]) < RES_TOL
+        off += f.n
+    bt.close()
+
+
 def test_batch_f32_refinement(torch):
     from rbsptrsv import Batch
     facs, nrhs = _small_batch()
