@@ -76,6 +76,16 @@ rb_status bfs_levels(const int *d_colptr, const int *d_rowidx,
                      const int *d_rowcnt, int m, int *d_levelItem,
                      int *d_levelPtr, int *nlevel, hipStream_t s);
 
+// The same level analysis for every triangular block of a block-diagonal
+// CSC at once, one workgroup per block: block t owns rows [ups[t],
+// ups[t] + bms[t]); its rows in discovery order go to item[ups[t] ..]
+// (permuted ids), its block-local level starts to lvptr[lvbase[t] ..]
+// (level 0 at 0) and its level count to nlv[t] (device arrays).
+rb_status bfs_blocks(const int *d_colptr, const int *d_rowidx,
+                     const int *d_rowcnt, int m, int nblocks, const int *d_ups,
+                     const int *d_bms, const int *d_lvbase, int *d_item,
+                     int *d_lvptr, int *d_nlv, hipStream_t s);
+
 // Per-block descriptor kept on the host (and mirrored on the device for the
 // fused executor).
 struct Block {

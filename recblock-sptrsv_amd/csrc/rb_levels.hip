@@ -426,6 +426,142 @@ k_bfs(const int *__restrict__ colptr, const int *__restrict__ rowidx, int m,
     }
 }
 
+// One level [s, e) of any width inside one workgroup: the A / B / scan /
+// C phases of the narrow levels, with the counts in device memory (cnt,
+// indexed by frontier position) and a scan in chunks of the workgroup's
+// width.  A thread handles the same positions in B and C, so it reads back
+// only its own counts.
+__device__ int level_wide(const int *__restrict__ colptr,
+                          const int *__restrict__ rowidx, int *rem, int *par,
+                          int *levelItem, int s, int e, int *cnt, int *lds) {
+    const int tid = threadIdx.x;
+    for (int p = s + tid; p < e; p += blockDim.x) {
+        const int u = ld_wg(&levelItem[p]);
+        for (int q = colptr[u]; q < colptr[u + 1]; q++) {
+            const int v = rowidx[q];
+            if (v == u) continue;
+            __hip_atomic_fetch_add(&rem[v], -1, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_max(&par[v], p, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    int carry = 0;
+    for (int base = s; base < e; base += blockDim.x) {  // uniform trip count
+        const int p = base + tid;
+        int c = 0;
+        if (p < e) {
+            const int u = ld_wg(&levelItem[p]);
+            for (int q = colptr[u]; q < colptr[u + 1]; q++) {
+                const int v = rowidx[q];
+                if (v != u && ld_wg(&rem[v]) == 0 && ld_wg(&par[v]) == p) c++;
+            }
+        }
+        int tot = 0;
+        const int off = block_excl_scan(c, lds, &tot);
+        if (p < e) cnt[p] = carry + off;
+        carry += tot;
+    }
+    for (int p = s + tid; p < e; p += blockDim.x) {
+        const int u = ld_wg(&levelItem[p]);
+        int o = e + cnt[p];
+        for (int q = colptr[u]; q < colptr[u + 1]; q++) {
+            const int v = rowidx[q];
+            if (v != u && ld_wg(&rem[v]) == 0 && ld_wg(&par[v]) == p)
+                __hip_atomic_store(&levelItem[o++], v, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    }
+    return carry;
+}
+
+// The level analysis of every triangular block at once, one workgroup per
+// block (blockIdx.x = t): the same discovery order as the whole-matrix
+// walk (R/findlevel.h:7-83), restricted to the block's own entries.  The
+// CSC holds the block-diagonal matrix (every row's in-block entries; row
+// ids are permuted ids, block t owns [ups[t], ups[t] + bms[t])), so the
+// blocks share no row and each workgroup reads and writes only its own
+// rows: no grid barrier, only workgroup barriers.  Writes the block's rows
+// in discovery order to item[ups[t] ..] (permuted ids), its level starts
+// (block-local, level 0 at 0) to lvptr[lvbase[t] ..] and its level count
+// to nlv[t]; a block that cannot finish (a row never released) sets
+// *stalled.
+__global__ void __launch_bounds__(kBfsThreads)
+k_bfs_blocks(const int *__restrict__ colptr, const int *__restrict__ rowidx,
+             const int *__restrict__ rowcnt, const int *__restrict__ ups,
+             const int *__restrict__ bms, const int *__restrict__ lvbase,
+             int *item, int *lvptr, int *nlv, int *rem, int *par, int *cnt,
+             int *stalled) {
+    __shared__ int lds[16];
+    __shared__ int cl[kNarrow];
+    const int t = blockIdx.x, tid = threadIdx.x;
+    const int up = ups[t], bm = bms[t];
+    int *lp = lvptr + lvbase[t];
+    if (bm == 0) {
+        if (tid == 0) {
+            lp[0] = 0;
+            nlv[t] = 0;
+        }
+        return;
+    }
+    int *it = item + up;
+    for (int i = tid; i < bm; i += blockDim.x) {
+        rem[up + i] = rowcnt[up + i] - 1;
+        par[up + i] = -1;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    // level 0: the rows without an in-block dependency, ascending
+    int e = 0;
+    for (int base = 0; base < bm; base += blockDim.x) {  // uniform trip count
+        const int i = base + tid;
+        const int f = i < bm && ld_wg(&rem[up + i]) == 0 ? 1 : 0;
+        int tot = 0;
+        const int off = block_excl_scan(f, lds, &tot);
+        if (f)
+            __hip_atomic_store(&it[e + off], up + i, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+        e += tot;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    if (tid == 0) {
+        lp[0] = 0;
+        lp[1] = e;
+    }
+    int s = 0, nl = 1;
+    bool bad = e == 0;
+    while (e < bm && !bad) {
+        const int len = e - s;
+        const int total =
+            len <= kIlp * kBfsThreads
+                ? level_cached(colptr, rowidx, rem, par, it, s, e, cl, lds)
+            : len <= kNarrow
+                ? level_generic(colptr, rowidx, rem, par, it, s, e, cl, lds)
+                : level_wide(colptr, rowidx, rem, par, it, s, e, cnt + up, lds);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        if (total == 0) {
+            bad = true;
+            break;
+        }
+        if (tid == 0) lp[nl + 1] = e + total;
+        nl++;
+        s = e;
+        e += total;
+    }
+    if (tid == 0) {
+        nlv[t] = nl;
+        if (bad) atomicOr(stalled, 1);
+    }
+}
+
 __global__ void k_bfs_init(const int *__restrict__ rowcnt, int m, int *rem,
                            int *par) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -436,6 +572,37 @@ __global__ void k_bfs_init(const int *__restrict__ rowcnt, int m, int *rem,
 }
 
 }  // namespace
+
+rb_status bfs_blocks(const int *d_colptr, const int *d_rowidx,
+                     const int *d_rowcnt, int m, int nblocks, const int *d_ups,
+                     const int *d_bms, const int *d_lvbase, int *d_item,
+                     int *d_lvptr, int *d_nlv, hipStream_t s) {
+    if (m == 0 || nblocks == 0) return RB_OK;
+    Arena tmp;
+    int *rem, *par, *cnt, *stalled;
+    RB_TRY(tmp.alloc(&rem, m));
+    RB_TRY(tmp.alloc(&par, m));
+    RB_TRY(tmp.alloc(&cnt, m));
+    RB_TRY(tmp.alloc(&stalled, 1));
+    RB_HIP(hipMemsetAsync(stalled, 0, sizeof(int), s));
+    k_bfs_blocks<<<nblocks, kBfsThreads, 0, s>>>(d_colptr, d_rowidx, d_rowcnt,
+                                                 d_ups, d_bms, d_lvbase, d_item,
+                                                 d_lvptr, d_nlv, rem, par, cnt,
+                                                 stalled);
+    RB_HIP(hipGetLastError());
+    int h = 0;
+    hipError_t e = hipMemcpyAsync(&h, stalled, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    tmp.release();
+    if (e != hipSuccess) return status_of_hip(e, "k_bfs_blocks");
+    if (h) {
+        set_error(RB_ERR_MISSING_DIAG,
+                  "block level analysis stalled: a row is never released "
+                  "(missing diagonal or cyclic dependency)");
+        return RB_ERR_MISSING_DIAG;
+    }
+    return RB_OK;
+}
 
 rb_status bfs_levels(const int *d_colptr, const int *d_rowidx,
                      const int *d_rowcnt, int m, int *d_levelItem,

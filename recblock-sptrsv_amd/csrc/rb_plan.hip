@@ -333,49 +333,14 @@ __global__ void k_drowcnt(Geom g, const DevBlock *__restrict__ blk,
 }
 
 // level of every node of D and its triangular block id (sort key)
-__global__ void k_node_level(Geom g, const int *__restrict__ item,
-                             const int *__restrict__ lptr, int nlev,
-                             int *key, int *lvl_of_pos) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= g.m) return;
-    int lo = 0, hi = nlev;  // largest L with lptr[L] <= p
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (lptr[mid] <= p) lo = mid;
-        else hi = mid;
-    }
-    const int v = item[p];
-    key[p] = block_of(g, v, v) >> 1;
-    lvl_of_pos[v] = lo;
-}
-
-// After the stable sort by block: item_sorted holds permuted ids grouped by
-// triangular block, in D's BFS order inside each group.
-__global__ void k_local_levels(Geom g, const int *__restrict__ item_sorted,
-                               const int *__restrict__ lvl_of,
-                               const int *__restrict__ tri_up,
-                               const int *__restrict__ tri_lvbase, int ntri,
-                               int *lvitem, int *lvptr, int *nlv) {
+// Block-local row ids of the blocks' discovery orders: position i of the
+// item array lies in the triangular block that owns row i (the blocks tile
+// the rows in order, each block's order stored at its first row).
+__global__ void k_local_ids(Geom g, const int *__restrict__ item,
+                            const int *__restrict__ tri_up, int *lvitem) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= g.m) return;
-    const int v = item_sorted[i];
-    const int t = block_of(g, v, v) >> 1;
-    const int up = tri_up[t];
-    const int L = lvl_of[v];
-    lvitem[i] = v - up;
-    const int prev_t = i > 0 ? (block_of(g, item_sorted[i - 1], item_sorted[i - 1]) >> 1) : -1;
-    const int prevL = i > 0 ? lvl_of[item_sorted[i - 1]] : -1;
-    (void)ntri;
-    if (prev_t != t || prevL != L) {
-        // first element of level L of block t: the raw index into the
-        // sorted array (rebased to the block's segment on the host)
-        lvptr[tri_lvbase[t] + L] = i;
-    }
-    // levels are non-decreasing inside a block's segment: its last element
-    // carries the block's level count
-    const int next_t =
-        i + 1 < g.m ? (block_of(g, item_sorted[i + 1], item_sorted[i + 1]) >> 1) : -1;
-    if (next_t != t) nlv[t] = L + 1;
+    lvitem[i] = item[i] - tri_up[block_of(g, i, i) >> 1];
 }
 
 // nnz_lv of find_level_nnz (R/utils_cuda.h:781-796): sum of row counts over
@@ -680,23 +645,18 @@ rb_status build_plan(rb_plan *p, const int *d_colptr, const int *d_rowidx,
         }
     }
 
-    // ---- 7. local level analysis of the triangular blocks (BFS of the
-    // block-diagonal matrix D; equals each block's own BFS, see header)
+    // ---- 7. local level analysis of the triangular blocks: each block's
+    // own BFS over its in-block entries (the block-diagonal matrix D), all
+    // blocks at once, one workgroup each
     const int ntri = p->ntri;
     {
-        int *dcnt, *dptr, *didx, *drc, *ditem, *dlptr, *key, *lvl_of,
-            *keys_sorted, *item_sorted;
+        int *dcnt, *dptr, *didx, *drc, *ditem;
         const int dnnz_max = nnz;
         RB_TRY(tmp.alloc(&dcnt, m + 1));
         RB_TRY(tmp.alloc(&dptr, m + 1));
         RB_TRY(tmp.alloc(&didx, dnnz_max));
         RB_TRY(tmp.alloc(&drc, m));
         RB_TRY(tmp.alloc(&ditem, m));
-        RB_TRY(tmp.alloc(&dlptr, m + 2));
-        RB_TRY(tmp.alloc(&key, m));
-        RB_TRY(tmp.alloc(&lvl_of, m));
-        RB_TRY(tmp.alloc(&keys_sorted, m));
-        RB_TRY(tmp.alloc(&item_sorted, m));
         if (m > 0) {
             k_dcol_count<<<nblk(m), kT, 0, s>>>(g, p->d_pcolptr, p->d_prowidx,
                                                 dcnt);
@@ -704,60 +664,47 @@ rb_status build_plan(rb_plan *p, const int *d_colptr, const int *d_rowidx,
         }
         RB_HIP(hipMemsetAsync(dcnt + m, 0, sizeof(int), s));
         RB_TRY(exclusive_scan_i32(dcnt, dptr, m + 1, s));
-        int dlev = 0;
+        // per-triangular-block bases
+        std::vector<int> tri_up(ntri), tri_m(ntri), tri_lvbase(ntri),
+            tri_sb(ntri);
+        int lb = 0;
+        for (int t = 0; t < ntri; t++) {
+            Block &b = p->blocks[2 * t];
+            tri_up[t] = b.up;
+            tri_m[t] = b.blk_m;
+            tri_lvbase[t] = lb;
+            tri_sb[t] = b.slot_base;
+            b.lvptr_base = lb;
+            b.lv_base = b.up;  // the blocks tile the rows in order
+            lb += b.blk_m + 1;
+        }
+        RB_TRY(A.alloc(&p->d_lvptr, lb));
+        RB_TRY(A.alloc(&p->d_lvitem, m));
+        RB_TRY(A.alloc(&p->d_nnzlv, lb));
+        int *d_triup, *d_trim, *d_trilvb, *d_nlv, *d_trisb;
+        RB_TRY(tmp.alloc(&d_triup, ntri));
+        RB_TRY(tmp.alloc(&d_trim, ntri));
+        RB_TRY(tmp.alloc(&d_trilvb, ntri));
+        RB_TRY(tmp.alloc(&d_nlv, ntri));
+        RB_TRY(tmp.alloc(&d_trisb, ntri));
+        RB_TRY(h2d(d_triup, tri_up, s));
+        RB_TRY(h2d(d_trim, tri_m, s));
+        RB_TRY(h2d(d_trilvb, tri_lvbase, s));
+        RB_TRY(h2d(d_trisb, tri_sb, s));
+        RB_HIP(hipMemsetAsync(d_nlv, 0, sizeof(int) * ntri, s));
         if (m > 0) {
             k_dcol_fill<<<nblk(m), kT, 0, s>>>(g, p->d_pcolptr, p->d_prowidx,
                                                dptr, didx);
             k_drowcnt<<<nblk(m), kT, 0, s>>>(g, d_blk, cnt, drc);
             RB_HIP(hipGetLastError());
-            RB_TRY(bfs_levels(dptr, didx, drc, m, ditem, dlptr, &dlev, s));
-            k_node_level<<<nblk(m), kT, 0, s>>>(g, ditem, dlptr, dlev, key,
-                                                lvl_of);
-            RB_HIP(hipGetLastError());
-            RB_TRY(sort_pairs_i32(key, keys_sorted, ditem, item_sorted, m,
-                                  bits_for(ntri), s));
-        }
-        // per-triangular-block bases
-        std::vector<int> tri_up(ntri), tri_lvbase(ntri), tri_itembase(ntri),
-            tri_sb(ntri);
-        int lb = 0, ib = 0;
-        for (int t = 0; t < ntri; t++) {
-            Block &b = p->blocks[2 * t];
-            tri_up[t] = b.up;
-            tri_lvbase[t] = lb;
-            tri_itembase[t] = ib;
-            tri_sb[t] = b.slot_base;
-            b.lvptr_base = lb;
-            b.lv_base = ib;
-            lb += b.blk_m + 1;
-            ib += b.blk_m;
-        }
-        RB_TRY(A.alloc(&p->d_lvptr, lb));
-        RB_TRY(A.alloc(&p->d_lvitem, m));
-        RB_TRY(A.alloc(&p->d_nnzlv, lb));
-        int *d_triup, *d_trilvb, *d_nlv, *d_triib, *d_trisb;
-        RB_TRY(tmp.alloc(&d_triup, ntri));
-        RB_TRY(tmp.alloc(&d_trilvb, ntri));
-        RB_TRY(tmp.alloc(&d_nlv, ntri));
-        RB_TRY(tmp.alloc(&d_triib, ntri));
-        RB_TRY(tmp.alloc(&d_trisb, ntri));
-        RB_TRY(h2d(d_triup, tri_up, s));
-        RB_TRY(h2d(d_trilvb, tri_lvbase, s));
-        RB_TRY(h2d(d_triib, tri_itembase, s));
-        RB_TRY(h2d(d_trisb, tri_sb, s));
-        RB_HIP(hipMemsetAsync(d_nlv, 0, sizeof(int) * ntri, s));
-        if (m > 0) {
-            k_local_levels<<<nblk(m), kT, 0, s>>>(g, item_sorted, lvl_of,
-                                                  d_triup, d_trilvb, ntri,
-                                                  p->d_lvitem, p->d_lvptr,
-                                                  d_nlv);
+            RB_TRY(bfs_blocks(dptr, didx, drc, m, ntri, d_triup, d_trim,
+                              d_trilvb, ditem, p->d_lvptr, d_nlv, s));
+            k_local_ids<<<nblk(m), kT, 0, s>>>(g, ditem, d_triup, p->d_lvitem);
             RB_HIP(hipGetLastError());
         }
         std::vector<int> hnlv;
         RB_TRY(d2h(hnlv, d_nlv, ntri, s));
-        // The level starts were written as raw indices into the sorted
-        // array; the segment of block t starts at its item base.  Rebase,
-        // and close each level table with blk_m.
+        // close each level table with blk_m
         std::vector<int> hlvptr;
         RB_TRY(d2h(hlvptr, p->d_lvptr, (size_t)lb, s));
         for (int t = 0; t < ntri; t++) {
@@ -765,7 +712,6 @@ rb_status build_plan(rb_plan *p, const int *d_colptr, const int *d_rowidx,
             int nl = hnlv[t];
             if (b.blk_m == 0) nl = 1;
             int *lp = hlvptr.data() + tri_lvbase[t];
-            for (int L = 0; L < nl && b.blk_m > 0; L++) lp[L] -= tri_itembase[t];
             lp[0] = 0;
             lp[nl] = b.blk_m;
             b.nlv = nl;
