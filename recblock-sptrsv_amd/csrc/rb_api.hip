@@ -41,17 +41,24 @@ static int lv_auto(int m, int device_id) {
     return li;
 }
 
-// Schedule choice.  BLOCKED and FUSED are taken as asked; AUTO times warm
-// solves of each candidate on scratch right-hand sides (tune_rhs columns
-// per call, default 1) and keeps the faster (the plan is built for both).
-static rb_status choose_schedule(rb_plan *p, hipStream_t s) {
+// Kernel choice per block (tune_blocks, rb_opts.block_kernels) and the
+// schedule.  BLOCKED and FUSED are taken as asked (a fused-only plan keeps
+// the reference selector's block kernels without timing them); AUTO times
+// warm solves of each candidate on scratch right-hand sides (tune_rhs
+// columns per call, default 1) and keeps the faster (the plan is built for
+// both).  The fused schedule is timed first: with one column, the block
+// kernel timing stops as soon as the timed blocks alone take more than
+// twice the fused solve, and the blocked schedule is then not timed at all;
+// otherwise its first timed solve settles the choice when it is more than
+// twice the fused time (its other two are then not run).
+static rb_status choose_schedule(rb_plan *p, int block_mode, hipStream_t s) {
     if (p->want_schedule == RB_SCHED_BLOCKED || !p->fused) {
         p->schedule = RB_SCHED_BLOCKED;
-        return RB_OK;
+        return tune_blocks(p, block_mode, s);
     }
     if (p->want_schedule == RB_SCHED_FUSED) {
         p->schedule = RB_SCHED_FUSED;
-        return RB_OK;
+        return tune_blocks(p, 1, s);
     }
     const int m = p->m;
     const int k = std::max(1, std::min(p->tune_rhs, 64));
@@ -68,53 +75,55 @@ static rb_status choose_schedule(rb_plan *p, hipStream_t s) {
     (void)hipEventCreate(&e1);
     // the work of a solve does not depend on the values: b = 0
     if (hipMemsetAsync(db, 0, bytes, s) != hipSuccess) st = RB_ERR_HIP;
-    // fused first; the blocked schedule's first timed solve settles the
-    // choice when it is more than twice the fused time (its other two are
-    // then not run)
-    double ms[2] = {0.0, 0.0};
-    for (int c = 1; c >= 0 && st == RB_OK; c--) {
-        auto run = [&]() {
-            return c == 0 ? solve_blocked(p, db, m, dx, m, k, s)
-                          : solve_fused(p, db, m, dx, m, k, s);
-        };
-        st = run();  // warm
-        if (st != RB_OK) break;
-        // timed solves: fused 3 at once; blocked 1, then 2 more if needed
-        auto timed = [&](int reps) {
-            float t = 0.f;
-            (void)hipEventRecord(e0, s);
-            for (int r = 0; r < reps && st == RB_OK; r++) st = run();
-            (void)hipEventRecord(e1, s);
-            (void)hipEventSynchronize(e1);
-            (void)hipEventElapsedTime(&t, e0, e1);
-            return (double)t;
-        };
-        if (c == 1) {
-            ms[1] = timed(3) / 3.0;
-        } else {
-            const double t1 = timed(1);
-            ms[0] = t1 > 2.0 * ms[1] || st != RB_OK ? t1 : (t1 + timed(2)) / 3.0;
+    auto timed = [&](int c, int reps) {
+        float t = 0.f;
+        (void)hipEventRecord(e0, s);
+        for (int r = 0; r < reps && st == RB_OK; r++)
+            st = c == 0 ? solve_blocked(p, db, m, dx, m, k, s)
+                        : solve_fused(p, db, m, dx, m, k, s);
+        (void)hipEventRecord(e1, s);
+        (void)hipEventSynchronize(e1);
+        (void)hipEventElapsedTime(&t, e0, e1);
+        return (double)t;
+    };
+    // a schedule whose solve tripped the watchdog is never chosen
+    auto tripped = [&]() {
+        int d = 0;
+        if (hipMemcpy(&d, p->d_status, sizeof(int), hipMemcpyDeviceToHost) !=
+            hipSuccess)
+            st = RB_ERR_HIP;
+        if (d != 0) {
+            const int z = 0;
+            (void)hipMemcpy(p->d_status, &z, sizeof(int), hipMemcpyHostToDevice);
+            fused_rearm(p);
         }
+        return d != 0;
+    };
+    double ms[2] = {0.0, 0.0};
+    bool ok[2] = {true, true}, cut = false;
+    if (st == RB_OK) {
+        (void)timed(1, 1);  // warm
+        ms[1] = timed(1, 3) / 3.0;
+        ok[1] = !tripped();
+    }
+    if (st == RB_OK)
+        st = tune_blocks(p, block_mode, s, k == 1 && ok[1] ? 2.0 * ms[1] : 0.0,
+                         &cut);
+    if (st == RB_OK && !cut) {
+        (void)timed(0, 1);  // warm (captures the walk)
+        const double t1 = timed(0, 1);
+        ms[0] = t1 > 2.0 * ms[1] && ok[1] ? t1 : (t1 + timed(0, 2)) / 3.0;
+        ok[0] = !tripped();
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipFree(db);
     (void)hipFree(dx);
-    int dstat = 0;
-    if (hipMemcpy(&dstat, p->d_status, sizeof(int), hipMemcpyDeviceToHost) !=
-        hipSuccess)
-        st = RB_ERR_HIP;
     if (st != RB_OK) return st;
-    p->tune_ms_blocked = ms[0];
+    p->tune_ms_blocked = cut ? 0.0 : ms[0];
     p->tune_ms_fused = ms[1];
-    // a fused solve that tripped the watchdog is never chosen
-    p->schedule = (dstat == 0 && ms[1] < ms[0]) ? RB_SCHED_FUSED
-                                                : RB_SCHED_BLOCKED;
-    if (dstat != 0) {
-        int z = 0;
-        (void)hipMemcpy(p->d_status, &z, sizeof(int), hipMemcpyHostToDevice);
-        fused_rearm(p);
-    }
+    p->schedule = ok[1] && (cut || !ok[0] || ms[1] < ms[0]) ? RB_SCHED_FUSED
+                                                            : RB_SCHED_BLOCKED;
     return RB_OK;
 }
 
@@ -207,10 +216,7 @@ rb_status rb_plan_create(const int *d_colptr, const int *d_rowidx,
         auto t0 = std::chrono::steady_clock::now();
         // kernels per block for the blocked walk (not timed when only the
         // fused schedule will run)
-        st = tune_blocks(p, o.schedule == RB_SCHED_FUSED && p->fused ? 1
-                                                                     : o.block_kernels,
-                         (hipStream_t)stream);
-        if (st == RB_OK) st = choose_schedule(p, (hipStream_t)stream);
+        st = choose_schedule(p, o.block_kernels, (hipStream_t)stream);
         p->phase_ms[3] = std::chrono::duration<double, std::milli>(
                              std::chrono::steady_clock::now() - t0)
                              .count();

@@ -801,15 +801,17 @@ static bool wait_graph_acyclic(int nblk, const std::vector<int> &step0, int NB,
     return (int)q.size() == nblk;
 }
 
-// Host partitioner and record builder.  rp/cc/cv: permuted CSR (ascending
-// columns); lp: BFS level boundaries; lf: final permutation; invd: 1/diag.
-rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
-                           const std::vector<int> &cc,
-                           const std::vector<double> &cv,
-                           const std::vector<int> &lp,
-                           const std::vector<int> &lf,
-                           const std::vector<double> &invd, int Wmax,
-                           hipStream_t s) {
+template <class T>
+using HostVec = std::vector<T, NoInit<T>>;
+
+// Host partitioner and record builder.  rp/cc: permuted CSR structure
+// (ascending columns); lp: BFS level boundaries.  The values, the final
+// permutation and 1/diag are fetched from the device (d_cval, the plan's
+// levelItem and invd) only once the schedule is known to be worth
+// building.
+rb_status build_fused_host(rb_plan *p, const HostVec<int> &rp,
+                           const HostVec<int> &cc, const HostVec<int> &lp,
+                           const double *d_cval, int Wmax, hipStream_t s) {
     auto t0 = std::chrono::steady_clock::now();
     // phase times for RB_FUSED_TRACE=1
     std::vector<std::pair<const char *, double>> marks;
@@ -831,8 +833,9 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
     const int nlev = p->nlevel;
 
     // ---- wave assignment (chain following) and steps
-    std::vector<int> wave(m), stepw(m), lanew(m);
-    std::vector<int> par(m, -1);
+    // every entry of these is written before it is read
+    HostVec<int> wave((size_t)std::max(1, m)), stepw((size_t)std::max(1, m)),
+        lanew((size_t)std::max(1, m)), par((size_t)std::max(1, m));
     // a row's parent: its latest dependency (rows split over threads)
     const int nth_h = std::max(1, std::min({16, (int)std::thread::hardware_concurrency(),
                                             (m + 65535) / 65536}));
@@ -898,7 +901,7 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
     }
     // steps: a wave's rows of one level, 64 at a time; lane of the parent
     // when it is free
-    std::vector<int> lvl(m);
+    HostVec<int> lvl((size_t)std::max(1, m));
     for (int L = 0; L < nlev; L++)
         for (int q = lp[L]; q < lp[L + 1]; q++) lvl[q] = L;
     std::vector<std::vector<std::array<int, 64>>> steps(W);
@@ -972,15 +975,19 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
     };
     const int nth_c = std::max(1, std::min({16, (int)std::thread::hardware_concurrency(),
                                             (m + 65535) / 65536}));
-    std::vector<Tally> tally(nth_c);
-    {
+    const bool may_skip = p->want_schedule == RB_SCHED_AUTO && p->tune_rhs <= 1;
+    // Rows q = 0, stride, 2*stride, ... only (stride 1: every row).  A
+    // sample of every 16th row settles a schedule that is clearly not
+    // worth building (more than 35 % of the sampled entries cross waves)
+    // without the full pass.
+    auto run_tally = [&](int stride, std::vector<Tally> &tally) {
         std::vector<std::thread> pool;
         for (int t = 0; t < nth_c; t++)
             pool.emplace_back([&, t]() {
                 Tally &c = tally[t];
                 const int q0 = (int)((long)m * t / nth_c);
                 const int q1 = (int)((long)m * (t + 1) / nth_c);
-                for (int q = q0; q < q1; q++) {
+                for (int q = (q0 + stride - 1) / stride * stride; q < q1; q += stride) {
                     const int r = pof(q);
                     int k_dpp = 0, k_all = 0;
                     for (int e = rp[r]; e < rp[r + 1]; e++) {
@@ -1002,7 +1009,28 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
                 }
             });
         for (auto &t : pool) t.join();
+    };
+    if (may_skip && m >= (1 << 20)) {
+        std::vector<Tally> sample(nth_c);
+        run_tally(16, sample);
+        long all = 0, loc = 0;
+        for (const Tally &c : sample) {
+            all += c.all;
+            loc += c.loc;
+        }
+        if (100 * (all - loc) > 35 * all) {
+            if (getenv("RB_FUSED_TRACE")) {
+                mark("sampled classes");
+                fprintf(stderr, "fused schedule not built: %ld of %ld sampled "
+                        "entries cross waves; build (cumulative ms):", all - loc, all);
+                for (auto &mk : marks) fprintf(stderr, " %s %.1f", mk.first, mk.second);
+                fprintf(stderr, "\n");
+            }
+            return RB_OK;
+        }
     }
+    std::vector<Tally> tally(nth_c);
+    run_tally(1, tally);
     Tally tot;
     for (const Tally &c : tally) {
         tot.all += c.all;
@@ -1020,9 +1048,30 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
     // When the caller leaves the choice to the tuner for single right-hand
     // sides, do not build it: its build is the most expensive part of the
     // plan on such factors (1.5 s of 1.7 s on the 4M-row power-law case).
-    if (p->want_schedule == RB_SCHED_AUTO && p->tune_rhs <= 1 &&
-        4 * (n_all - n_loc) > n_all)
+    if (may_skip && 4 * (n_all - n_loc) > n_all) {
+        if (getenv("RB_FUSED_TRACE")) {
+            mark("classes");
+            fprintf(stderr, "fused schedule not built: %ld of %ld entries cross "
+                    "waves; build (cumulative ms):", n_all - n_loc, n_all);
+            for (auto &mk : marks) fprintf(stderr, " %s %.1f", mk.first, mk.second);
+            fprintf(stderr, "\n");
+        }
         return RB_OK;
+    }
+    // the rest of the tables: values, permutation, 1/diag
+    HostVec<double> cv((size_t)std::max(1, p->nnz)), invd((size_t)std::max(1, m));
+    HostVec<int> lf((size_t)std::max(1, m));
+    if (p->nnz > 0)
+        RB_HIP(hipMemcpyAsync(cv.data(), d_cval, sizeof(double) * p->nnz,
+                              hipMemcpyDeviceToHost, s));
+    if (m > 0) {
+        RB_HIP(hipMemcpyAsync(lf.data(), p->d_levelItem, sizeof(int) * m,
+                              hipMemcpyDeviceToHost, s));
+        RB_HIP(hipMemcpyAsync(invd.data(), p->d_invd, sizeof(double) * m,
+                              hipMemcpyDeviceToHost, s));
+    }
+    RB_HIP(hipStreamSynchronize(s));
+    mark("values");
     const long n_lo = tot.lo, n_up = tot.up;
     const int NDPP = !(n_dpp > 0 && 4 * n_dpp >= n_all) ? 0
                    : n_up == 0 ? 1 : n_lo == 0 ? 2 : 3;
@@ -1265,39 +1314,29 @@ rb_status build_fused_host(rb_plan *p, const std::vector<int> &rp,
     return ensure_rhs(F, 1, s);
 }
 
-// Device part of the fused build: pull the permuted CSR and the tables the
-// host partitioner needs.
+// Device part of the fused build: pull the permuted CSR's structure and
+// the level boundaries the host partitioner needs first.
 rb_status build_fused(rb_plan *p, const int *d_prptr, const int *d_ccol,
                       const double *d_cval, int Wmax, hipStream_t s) {
     const int m = p->m, nnz = p->nnz;
     auto tp = std::chrono::steady_clock::now();
-    std::vector<int> rp(m + 1), cc(nnz > 0 ? nnz : 1), lp(p->nlevel + 1),
-        lf(m > 0 ? m : 1);
-    std::vector<double> cv(nnz > 0 ? nnz : 1), invd(m > 0 ? m : 1);
+    HostVec<int> rp((size_t)m + 1), cc((size_t)std::max(1, nnz)),
+        lp((size_t)p->nlevel + 1);
     RB_HIP(hipMemcpyAsync(rp.data(), d_prptr, sizeof(int) * (m + 1),
                           hipMemcpyDeviceToHost, s));
-    if (nnz > 0) {
+    if (nnz > 0)
         RB_HIP(hipMemcpyAsync(cc.data(), d_ccol, sizeof(int) * nnz,
                               hipMemcpyDeviceToHost, s));
-        RB_HIP(hipMemcpyAsync(cv.data(), d_cval, sizeof(double) * nnz,
-                              hipMemcpyDeviceToHost, s));
-    }
     RB_HIP(hipMemcpyAsync(lp.data(), p->d_levelPtr,
                           sizeof(int) * (p->nlevel + 1), hipMemcpyDeviceToHost,
                           s));
-    if (m > 0) {
-        RB_HIP(hipMemcpyAsync(lf.data(), p->d_levelItem, sizeof(int) * m,
-                              hipMemcpyDeviceToHost, s));
-        RB_HIP(hipMemcpyAsync(invd.data(), p->d_invd, sizeof(double) * m,
-                              hipMemcpyDeviceToHost, s));
-    }
     RB_HIP(hipStreamSynchronize(s));
     if (getenv("RB_FUSED_TRACE"))
-        fprintf(stderr, "fused build: host copies of the tables %.1f ms\n",
+        fprintf(stderr, "fused build: host copies of the structure %.1f ms\n",
                 std::chrono::duration<double, std::milli>(
                     std::chrono::steady_clock::now() - tp)
                     .count());
-    return build_fused_host(p, rp, cc, cv, lp, lf, invd, Wmax, s);
+    return build_fused_host(p, rp, cc, lp, d_cval, Wmax, s);
 }
 
 int fused_max_waves() {

@@ -225,7 +225,11 @@ rb_status solve_f32ir(rb_plan *p, const double *d_b, int ldb, double *d_x,
                       hipStream_t s, int *iters, double *relres);
 // per-block kernel choice of the blocked walk by timing each candidate on
 // the plan's own buffers (rb_solve.hip); mode: rb_opts.block_kernels
-rb_status tune_blocks(rb_plan *p, int mode, hipStream_t s);
+// budget_ms > 0: stop timing once the timed blocks' best times add up to
+// more than budget_ms (the blocked walk cannot be faster; the remaining
+// blocks keep the rule) and set *cut
+rb_status tune_blocks(rb_plan *p, int mode, hipStream_t s,
+                      double budget_ms = 0.0, bool *cut = nullptr);
 // fused schedule (rb_fused.hip)
 rb_status build_fused(rb_plan *p, const int *d_prptr, const int *d_ccol,
                       const double *d_cval, int Wmax, hipStream_t s);

@@ -960,7 +960,9 @@ size_t blocked_ws_bytes(const rb_plan *p) {
 // triangular block on the sync-free / one-workgroup kernel (where it
 // applies).  The captured block walk is dropped so the next solve captures
 // the chosen kernels.
-rb_status tune_blocks(rb_plan *p, int mode, hipStream_t s) {
+rb_status tune_blocks(rb_plan *p, int mode, hipStream_t s, double budget_ms,
+                      bool *cut) {
+    if (cut) *cut = false;
     const int nb = p->nblocks, ntri = p->ntri;
     for (int i = 0; i < nb; i++) {
         Block &B = p->blocks[i];
@@ -1055,7 +1057,12 @@ rb_status tune_blocks(rb_plan *p, int mode, hipStream_t s) {
     // blocks below this many entries keep the rule (their launches cost
     // more than any difference between kernels)
     constexpr int kTuneMinEntries = 1 << 14;
+    double lower = 0.0;  // sum of the timed blocks' best times
     for (int i = 0; i < nb && st == RB_OK; i++) {
+        if (budget_ms > 0.0 && lower > budget_ms) {
+            if (cut) *cut = true;
+            break;
+        }
         Block &B = p->blocks[i];
         if (B.blk_m == 0 || B.blk_nnz < kTuneMinEntries) continue;
         std::vector<std::pair<int, int>> cand;  // (kern, lanes)
@@ -1087,6 +1094,7 @@ rb_status tune_blocks(rb_plan *p, int mode, hipStream_t s) {
         B.kern = pick.first;
         if (B.kind == 1) B.lanes = pick.second;
         B.tuned_ms = best;
+        if (best < 1e30) lower += best;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
