@@ -189,7 +189,15 @@ __global__ void k_tri_level_wave(const int *__restrict__ rowptr,
     if (lane == 0) st_x(&xperm[up + r], (bperm[up + r] - acc) * invd[up + r]);
 }
 
-// sync-free CSR pull, lane per row, 64 consecutive rows per wave ticket
+// sync-free CSR pull, lane per row, 64 consecutive rows per wave ticket.
+// The grid is one-dimensional with the columns interleaved: workgroup i
+// solves column i % width, so consecutive workgroups are the columns of one
+// tile and every column's frontier is resident at once.  (With a grid of
+// (tiles, columns) the device dispatched all of column 0's tiles first;
+// they waited on their level chain while later columns waited for slots,
+// and a 32-column walk of the batch union ran its columns a few at a time.)
+// Within a column the tickets order the tiles, so every wait is on a tile
+// that a workgroup already holds.
 template <class T>
 __global__ void __launch_bounds__(kT)
 k_tri_syncfree(const int *__restrict__ rowptr, const int *__restrict__ col,
@@ -197,18 +205,19 @@ k_tri_syncfree(const int *__restrict__ rowptr, const int *__restrict__ col,
                const T *__restrict__ bperm, int sb, int up, int bm,
                int subst, int *ticket, T *xperm, int *status,
                unsigned spin_limit, long ld, int tstride, const int *__restrict__ ncolp,
-               const int *__restrict__ rowcap) {
-    if ((int)blockIdx.y >= *ncolp) return;
-    bperm += blockIdx.y * ld;
-    xperm += blockIdx.y * ld;
-    ticket += blockIdx.y * tstride;
+               const int *__restrict__ rowcap, int width) {
+    const int c = (int)(blockIdx.x % (unsigned)width);
+    if (c >= *ncolp) return;
+    bperm += c * ld;
+    xperm += c * ld;
+    ticket += c * tstride;
     const int lane = threadIdx.x & 63;
     int tile = 0;
     if (lane == 0) tile = atomicAdd(ticket, 1);
     tile = __shfl(tile, 0, 64);
     const int pos = tile * 64 + lane;
     const int r = subst == 0 ? pos : bm - 1 - pos;
-    const bool valid = pos < bm && !skip_row(rowcap, up + r);
+    const bool valid = pos < bm && !(rowcap && c >= rowcap[up + r]);
     int j = 0, e = 0;
     T bv = 0, iv = 0;
     if (valid) {
@@ -696,10 +705,10 @@ static void launch_block(rb_plan *p, int i, const T *val, const T *invd,
                         B.slot_base, B.up, xperm, ld, nc, p->d_rowcap);
             }
         } else {  // methods 1 and 3, or kTriSyncfree
-            k_tri_syncfree<T><<<dim3(nblk((long)B.blk_m, 64), ncol), 64, 0, s>>>(
+            k_tri_syncfree<T><<<(unsigned)nblk((long)B.blk_m, 64) * ncol, 64, 0, s>>>(
                 p->d_rowptr, p->d_col, val, invd, bperm, B.slot_base,
                 B.up, B.blk_m, p->subst, tickets + i / 2, xperm,
-                p->d_status, p->spin_limit, ld, p->ntri, nc, p->d_rowcap);
+                p->d_status, p->spin_limit, ld, p->ntri, nc, p->d_rowcap, ncol);
         }
         return;
     }
