@@ -42,16 +42,21 @@ class Server:
     ``Plan.from_factor``).  ``mode``: "fp64" (plain solves) or "f32ir" (FP32
     solves refined in FP64 to ``tol``).  ``mem_budget``: device bytes for
     request buffers in flight.  ``streams``: streams the factors are spread
-    over (default priority).  ``log``: called with a progress line after
-    each plan is built and warmed up."""
+    over (default priority).  ``max_iter``: corrections queued per f32ir
+    batch (the loop runs on the device without a host check; corrections
+    after convergence replay empty walks, so the count is kept near the
+    usual two; a column that needs more than max_iter makes check()
+    raise).  ``log``: called with a progress line after each plan is built
+    and warmed up."""
 
     def __init__(self, factors, mode="fp64", tol=1e-12, max_batch=64,
-                 mem_budget=1 << 30, streams=4, lv=-1, tune_rhs=8, log=None):
+                 mem_budget=1 << 30, streams=4, lv=-1, tune_rhs=8, log=None,
+                 max_iter=4):
         import torch
         if mode not in ("fp64", "f32ir"):
             raise ValueError("mode is fp64 or f32ir")
         self.torch = torch
-        self.mode, self.tol = mode, tol
+        self.mode, self.tol, self.max_iter = mode, tol, max_iter
         self.max_batch, self.mem_budget = max_batch, mem_budget
         self.log = log
         self.plans = []
@@ -106,7 +111,8 @@ class Server:
             else:
                 # no report: the loop runs on the device without host
                 # synchronisation; a column short of tol shows in check()
-                self.plans[f].solve_f32ir(B, X, tol=self.tol, stream=st,
+                self.plans[f].solve_f32ir(B, X, tol=self.tol,
+                                          max_iter=self.max_iter, stream=st,
                                           report=False)
             ev = torch.cuda.Event(enable_timing=True)
             ev.record(st)
