@@ -190,6 +190,34 @@ def test_multi_rhs_and_leading_dimension(oracle, torch, schedule, k):
         assert np.all(Xh[j, f.n:] == 0.0)   # padding rows untouched
 
 
+@pytest.mark.parametrize("kernels", [1, 2, 3])
+@pytest.mark.parametrize("name,make", [
+    ("powerlaw60000", lambda: gen.powerlaw(60000, seed=3)),
+    ("banded20000-upper", lambda: gen.banded(20000, seed=6).transpose()),
+])
+def test_blocked_columns_match_single(torch, kernels, name, make):
+    """The blocked walk solves k columns at once on a grid of the next
+    power of two (graphs kept per width, the column count read from device
+    memory): every column equals the single-column solve bit for bit, for
+    widths inside and across the buckets, in rising and falling order, with
+    each triangular kernel (reference selector's, sync-free with its
+    interleaved columns, one-workgroup)."""
+    f = make()
+    _say("blocked columns", name, kernels)
+    p = _plan(f, 3, schedule=1, block_kernels=kernels)
+    rng = np.random.default_rng(4)
+    B = rng.uniform(-1, 1, (33, f.n))
+    single = [p.solve(torch.as_tensor(B[j], device="cuda")).cpu().numpy()
+              for j in range(33)]
+    for k in (3, 17, 33, 5, 2, 32, 1):
+        X = p.solve(torch.as_tensor(B[:k], device="cuda").t()).t().cpu().numpy()
+        for j in range(k):
+            assert np.array_equal(X[j].view(np.int64), single[j].view(np.int64)), (k, j)
+    p.check()
+    assert residual(f, single[0], B[0]) < RES_TOL
+    p.close()
+
+
 @pytest.mark.parametrize("schedule", [0, 1, 2])
 def test_plan_build_phases(torch, schedule):
     """The plan reports its build phases: non-negative, the first three
