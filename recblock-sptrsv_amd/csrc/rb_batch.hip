@@ -9,7 +9,9 @@
 // columns of a union block of kmax = max_f nrhs_f columns: a gather pass
 // copies factor f's columns into its row range (zeros past nrhs_f), the
 // union plan solves all kmax columns (several per launch on the fused
-// schedule), and a scatter pass copies the solutions back.
+// schedule; the blocked walk skips each factor's rows past nrhs_f through
+// the per-row column counts written for the call), and a scatter pass
+// copies the solutions back.
 #include <algorithm>
 #include <cstring>
 #include <vector>

@@ -19,11 +19,24 @@
 //   at or above the block's mean row length (the reference uses one thread
 //   or one warp), plus a workgroup per row longer than 2048 entries (the
 //   long-row pass :30-58).
-// Every sum runs in a fixed order, so a solve is bitwise reproducible.
+// Besides the reference selector's kernels a triangular block may run the
+// sync-free kernel or a one-workgroup kernel that keeps recent x in an LDS
+// ring; at plan build, tune_blocks times the candidates of every large
+// block (and the lanes per row of every large square block) and keeps the
+// fastest.  Every sum runs in a fixed order, so a solve is bitwise
+// reproducible, and the kernels of one block agree bit for bit whenever
+// they read a row's entries in the same order.
+//
+// One walk solves up to 32 right-hand sides: every kernel takes its column
+// from the grid and the walk's column count from device memory, so a graph
+// captured for a grid width (a power of two) serves every count up to it.
+// Batches of factors give each row the column count of its own factor; the
+// kernels leave the row alone in later columns.
 //
 // The kernels are templates on the value type: the FP64 walk is the plain
 // solve; the FP32 walk (FP32 copies of the slot values) is the inner solver
-// of the FP32 + FP64 iterative refinement at the end of this file.
+// of the FP32 + FP64 iterative refinement at the end of this file, whose
+// correction loop runs on the device.
 #include <algorithm>
 #include <utility>
 #include <vector>
