@@ -191,7 +191,22 @@ __device__ int level_cached(const int *__restrict__ colptr,
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __syncthreads();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    // B: children (rem 0, parent p) per node
+    // B: children (rem 0, parent p) per node.  The cached entries' rem and
+    // par are read together (a non-candidate reads row 0, in range, and is
+    // ignored).  A narrow level costs about five dependent trips to memory
+    // (frontier, column pointers, entries, atomics, rem / par) at ~2 us
+    // each on the 1500^2 Laplacian; reading rem / par entry by entry behind
+    // branches cost another ~0.5 us.
+    int rv[kIlp][kE], pv[kIlp][kE];
+#pragma unroll
+    for (int k = 0; k < kIlp; k++)
+#pragma unroll
+        for (int j = 0; j < kE; j++) {
+            const int v = vc[k][j];
+            const int va = v >= 0 && v != u[k] ? v : 0;
+            rv[k][j] = ld_wg(&rem[va]);
+            pv[k][j] = ld_wg(&par[va]);
+        }
 #pragma unroll
     for (int k = 0; k < kIlp; k++) {
         const int i = tid + k * kBfsThreads;
@@ -202,9 +217,7 @@ __device__ int level_cached(const int *__restrict__ colptr,
         for (int j = 0; j < kE; j++) {
             const int v = vc[k][j];
             const bool cand = v >= 0 && v != u[k];
-            const int rv = cand ? ld_wg(&rem[v]) : 1;
-            const int pv = cand ? ld_wg(&par[v]) : -1;
-            if (cand && rv == 0 && pv == p) {
+            if (cand && rv[k][j] == 0 && pv[k][j] == p) {
                 c++;
                 bits |= 1u << j;
             }
