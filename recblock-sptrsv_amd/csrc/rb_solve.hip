@@ -586,31 +586,113 @@ __global__ void k_ir_update(const float *__restrict__ d, int m, double *x,
 }
 
 // r = b - A x in the original ordering, FP64: row i of the level-ordered
-// CSR is original row lf[i], its columns hold original ids; max |r|
-__global__ void k_ir_residual(const int *__restrict__ rptr,
-                              const int *__restrict__ rcol,
-                              const double *__restrict__ rval,
-                              const int *__restrict__ lf, int m,
-                              const double *__restrict__ b, long ldb,
-                              const double *__restrict__ x, long ldx,
-                              double *r, unsigned long long *rmax,
-                              const int *__restrict__ done) {
-    const int c = blockIdx.y;
-    if (done[c]) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double a = 0.0;
-    if (i < m) {
-        const int o = lf[i];
-        const double *xc = x + c * ldx;
-        double acc = 0.0;
-        for (int k = rptr[i]; k < rptr[i + 1]; k++) acc += rval[k] * xc[rcol[k]];
-        const double v = b[c * ldb + o] - acc;
-        r[(long)c * m + o] = v;
-        a = fabs(v);
+// CSR is original row lf[i], its columns hold original ids; max |r| per
+// column.  G lanes per row (a power of two at or above the mean row
+// length, as for the square blocks' SpMV) read the row's entries coalesced
+// and sum them in a fixed order (a thread per row was slow on the
+// power-law factor, whose rows reach 512 entries), for KC columns at once
+// (blockIdx.y * KC ..), so the matrix is read once per KC columns instead
+// of once per column.  The grid is at most 1024 workgroups per column
+// group, each looping over rows and folding its maxima into one atomic per
+// column: an atomic per wave on a column's one max word serialised (half a
+// million of them per correction with 8 lanes per row on 4M rows).
+template <int G, int KC>
+__global__ void __launch_bounds__(kT)
+k_ir_residual(const int *__restrict__ rptr, const int *__restrict__ rcol,
+              const double *__restrict__ rval, const int *__restrict__ lf,
+              int m, int ncol, const double *__restrict__ b, long ldb,
+              const double *__restrict__ x, long ldx, double *r,
+              unsigned long long *rmax, const int *__restrict__ done) {
+    const int c0 = blockIdx.y * KC;
+    bool live[KC];
+    bool any = false;
+#pragma unroll
+    for (int q = 0; q < KC; q++) {
+        live[q] = c0 + q < ncol && !done[c0 + q];
+        any = any || live[q];
     }
-    a = wave_max(a);
-    if ((threadIdx.x & 63) == 0)
-        atomicMax(&rmax[c], (unsigned long long)__double_as_longlong(a));
+    if (!any) return;  // the whole workgroup
+    __shared__ double wmax[KC][kT / 64];
+    const int g = threadIdx.x & (G - 1);
+    const long rows_per_pass = (long)gridDim.x * (blockDim.x / G);
+    double a[KC];
+#pragma unroll
+    for (int q = 0; q < KC; q++) a[q] = 0.0;
+    // i is the same for the G lanes of a group, so they loop together and
+    // their shuffles stay inside the group
+    for (long i = ((long)blockIdx.x * blockDim.x + threadIdx.x) / G; i < m;
+         i += rows_per_pass) {
+        double acc[KC];
+#pragma unroll
+        for (int q = 0; q < KC; q++) acc[q] = 0.0;
+        for (int k = rptr[i] + g; k < rptr[i + 1]; k += G) {
+            const double v = rval[k];
+            const int cj = rcol[k];
+#pragma unroll
+            for (int q = 0; q < KC; q++)
+                if (live[q]) acc[q] += v * x[(c0 + q) * ldx + cj];
+        }
+#pragma unroll
+        for (int q = 0; q < KC; q++) {
+#pragma unroll
+            for (int o = G / 2; o > 0; o >>= 1) acc[q] += __shfl_xor(acc[q], o, G);
+        }
+        if (g == 0) {
+            const int o = lf[i];
+#pragma unroll
+            for (int q = 0; q < KC; q++)
+                if (live[q]) {
+                    const int c = c0 + q;
+                    const double v = b[c * ldb + o] - acc[q];
+                    r[(long)c * m + o] = v;
+                    a[q] = fmax(a[q], fabs(v));
+                }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < KC; q++) {
+        const double w = wave_max(a[q]);
+        if ((threadIdx.x & 63) == 0) wmax[q][threadIdx.x >> 6] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x < KC && live[threadIdx.x]) {
+        double t = 0.0;
+        for (int w = 0; w < kT / 64; w++) t = fmax(t, wmax[threadIdx.x][w]);
+        atomicMax(&rmax[c0 + threadIdx.x], (unsigned long long)__double_as_longlong(t));
+    }
+}
+
+template <int G, int KC>
+static void launch_residual(rb_plan *p, int nc, const double *b, long ldb,
+                            const double *x, long ldx, hipStream_t s) {
+    const int m = p->m;
+    const int grid = (int)std::min<long>(1024, nblk((long)m * G));
+    k_ir_residual<G, KC><<<dim3(grid, (nc + KC - 1) / KC), kT, 0, s>>>(
+        p->d_rptr, p->d_rcol, p->d_rval, p->d_levelItem, m, nc, b, ldb, x,
+        ldx, p->d_r, p->d_norms + nc, p->d_irdone);
+}
+
+template <int KC>
+static void residual_kc(rb_plan *p, int nc, const double *b, long ldb,
+                        const double *x, long ldx, hipStream_t s) {
+    switch (lanes_per_row(p->m > 0 ? (int)((long)p->nnz / p->m) : 1)) {
+    case 1: launch_residual<1, KC>(p, nc, b, ldb, x, ldx, s); break;
+    case 2: launch_residual<2, KC>(p, nc, b, ldb, x, ldx, s); break;
+    case 4: launch_residual<4, KC>(p, nc, b, ldb, x, ldx, s); break;
+    case 8: launch_residual<8, KC>(p, nc, b, ldb, x, ldx, s); break;
+    case 16: launch_residual<16, KC>(p, nc, b, ldb, x, ldx, s); break;
+    case 32: launch_residual<32, KC>(p, nc, b, ldb, x, ldx, s); break;
+    default: launch_residual<64, KC>(p, nc, b, ldb, x, ldx, s); break;
+    }
+}
+
+static void residual(rb_plan *p, int nc, const double *b, long ldb,
+                     const double *x, long ldx, hipStream_t s) {
+    switch (cols_per_thread(nc)) {
+    case 4: residual_kc<4>(p, nc, b, ldb, x, ldx, s); break;
+    case 2: residual_kc<2>(p, nc, b, ldb, x, ldx, s); break;
+    default: residual_kc<1>(p, nc, b, ldb, x, ldx, s); break;
+    }
 }
 
 // after correction `it`: a column whose max |r| <= tol * max |b| is done;
@@ -1211,10 +1293,7 @@ rb_status solve_f32ir(rb_plan *p, const double *d_b, int ldb, double *d_x,
             // prologue sets the width its kernels read to zero)
             RB_TRY(walk<float>(p, nc, p->d_r32, m, p->d_d32, m, s, p->d_irleft));
             k_ir_update<<<g2, kT, 0, s>>>(p->d_d32, m, x, ldx, p->d_irdone);
-            k_ir_residual<<<g2, kT, 0, s>>>(p->d_rptr, p->d_rcol, p->d_rval,
-                                            p->d_levelItem, m, b, ldb, x, ldx,
-                                            p->d_r, p->d_norms + nc,
-                                            p->d_irdone);
+            residual(p, nc, b, ldb, x, ldx, s);
             k_ir_check<<<nblk(nc), kT, 0, s>>>(p->d_norms, p->d_norms + nc, nc,
                                                tol, p->d_irdone, p->d_iriters,
                                                p->d_irrel, it, p->d_irleft);
