@@ -173,19 +173,22 @@ __device__ inline __amdgpu_buffer_rsrc_t buf_rsrc(const void *p, long bytes) {
         const_cast<void *>(p), (short)0, (int)min(bytes, 0x7fffffffL), kBufWord3);
 }
 
-struct FusedArgs {
+// T: the value type of the solve (FP64, or FP32 for the inner solve of the
+// FP32 + FP64 refinement)
+template <class T>
+struct FusedArgsT {
     const int *step0;    // [W+1] first step of each wave
     const int *gent0;    // [W+1] first global entry of each wave
     const int *pub0;     // [W+1] first published slot of each wave
     const int4 *rec;     // [nsteps * K * 64] step records (b filled in)
     const int *gsrc;     // [nent] slot of the dependency
-    const double *gval;  // [nent]
+    const T *gval;       // [nent]
     const int *gmeta;    // [nent] group ordinal * 2 + last-of-group
     const int *pub;      // [npub] published slots, grouped by wave
-    double *xq;          // [nr][nsteps * 64] x in slot order (this solve)
-    double *xq_next;     // the other buffers, re-armed for the next solve
+    T *xq;               // [nr][nsteps * 64] x in slot order (this solve)
+    T *xq_next;          // the other buffers, re-armed for the next solve
                          // (null: nothing to re-arm)
-    const double *brec;  // [nr][nsteps * 64] b/diag in slot order
+    const T *brec;       // [nr][nsteps * 64] b/diag in slot order
     long nslots;         // nsteps * 64: stride between right-hand sides
     int W;               // consumer waves per right-hand side
     int *status;
@@ -195,7 +198,11 @@ struct FusedArgs {
     // [7] resolver FIFO-space wait, [8] resolver poll, [9] batches,
     // [10] poll rounds
     unsigned long long *trace;
+    // optional: the launch does nothing when *left == 0 (a refinement
+    // whose columns have all converged)
+    const int *left;
 };
+using FusedArgs = FusedArgsT<double>;
 
 __device__ inline unsigned long long now() { return wall_clock64(); }
 
@@ -211,6 +218,71 @@ __device__ inline double dpp_from_upper(double v) {
     const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x130, 0xf, 0xf, true);
     return __hiloint2double(hi, lo);
 }
+
+// Per value type: record coefficients, the LDS ring offsets, the DPP moves,
+// the sentinel and the loads and stores of x and b/diag.  The FP32 solve
+// reads the FP64 records and rounds each coefficient where it is used
+// (off the dependency chain): a record with an FP32 coefficient in half of
+// each pair would leave the other half of a loaded register pair unused,
+// and the register allocator then reuses it while the load is in flight
+// and waits for that load on every step.  Ring offsets are stored for
+// 8-byte slots and halved for 4-byte ones.
+template <class T> struct FV;
+template <> struct FV<double> {
+    using U = unsigned long long;
+    static constexpr U kPend = kPending;
+    __device__ static double coef(int lo, int hi) { return dbl(lo, hi); }
+    __device__ static int ring_off(int o) { return o; }
+    __device__ static double from_lower(double v) { return dpp_from_lower(v); }
+    __device__ static double from_upper(double v) { return dpp_from_upper(v); }
+    __device__ static bool pend(double v) { return pending(v); }
+    __device__ static double ld(const double *p) { return ld_x(p); }
+    __device__ static double fma(double a, double b, double c) {
+        return __builtin_fma(a, b, c);
+    }
+    __device__ static double load(__amdgpu_buffer_rsrc_t r, int off, int soff) {
+        return __builtin_bit_cast(double,
+                                  __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0));
+    }
+    template <int AUX>
+    __device__ static void store(double v, __amdgpu_buffer_rsrc_t r, int off,
+                                 int soff) {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, off,
+                                              soff, AUX);
+    }
+};
+template <> struct FV<float> {
+    using U = unsigned int;
+    static constexpr U kPend = kPending32;
+    __device__ static float coef(int lo, int hi) { return (float)dbl(lo, hi); }
+    __device__ static int ring_off(int o) { return o >> 1; }
+    __device__ static float from_lower(float v) {
+        return __int_as_float(
+            __builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
+    }
+    __device__ static float from_upper(float v) {
+        return __int_as_float(
+            __builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
+    }
+    __device__ static bool pend(float v) { return __float_as_uint(v) == kPending32; }
+    __device__ static float ld(const float *p) {
+        return __uint_as_float(__hip_atomic_load((const unsigned *)p, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT));
+    }
+    __device__ static float fma(float a, float b, float c) {
+        return __builtin_fmaf(a, b, c);
+    }
+    __device__ static float load(__amdgpu_buffer_rsrc_t r, int off, int soff) {
+        return __builtin_bit_cast(float,
+                                  __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0));
+    }
+    template <int AUX>
+    __device__ static void store(float v, __amdgpu_buffer_rsrc_t r, int off,
+                                 int soff) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off,
+                                              soff, AUX);
+    }
+};
 
 struct Shared {
     int ready;    // steps whose groups are all in the FIFO (resolver)
@@ -233,44 +305,46 @@ struct Shared {
 // block waits, directly or through other blocks, for a row of a later
 // step of its own block -- the host checks the block wait graph for cycles
 // and otherwise builds with BLK off, which checks the FIFO at every step.
-template <int NDPP, int NRING, bool BLK>
+template <int NDPP, int NRING, bool BLK, class T>
 struct Consumer {
     using F = Fmt<NDPP, NRING>;
     static constexpr int K = F::K, D = F::D, NB = F::NB;
+    static constexpr int kStep = 64 * (int)sizeof(T);  // bytes of x per step
 
-    const FusedArgs &a;
+    const FusedArgsT<T> &a;
     const int lane, ns, last;
-    const int lane16, lane8;  // byte offsets of the lane's record / x
+    const int lane16, laneB;  // byte offsets of the lane's record / x
     // this wave's records, slot-order x and b/diag
     const __amdgpu_buffer_rsrc_t rec, xq, br;
-    double *xring;
-    double *fifo;
+    T *xring;
+    T *fifo;
     Shared &sh;
     int4 rb[NB][K > 0 ? K : 1];  // ring of step records
-    double bb[NB];               // ring of b/diag
+    T bb[NB];                    // ring of b/diag
     int fifo_avail = 0;          // steps ready (scalar)
-    double fq[NB];               // FIFO values of the block's steps (BLK)
-    double xprev = 0.0;
+    T fq[NB];                    // FIFO values of the block's steps (BLK)
+    T xprev = 0;
     bool ok = true;
     unsigned long long t_wait = 0;
 
-    __device__ __forceinline__ Consumer(const FusedArgs &a_, int w, int r,
-                                        int lane_, double *xring_,
-                                        double *fifo_, Shared &sh_)
+    __device__ __forceinline__ Consumer(const FusedArgsT<T> &a_, int w, int r,
+                                        int lane_, T *xring_, T *fifo_,
+                                        Shared &sh_)
         : a(a_), lane(lane_), ns(a_.step0[w + 1] - a_.step0[w]),
           last(a_.step0[w + 1] - a_.step0[w] - 1),
-          lane16(lane_ * 16), lane8(lane_ * 8),
+          lane16(lane_ * 16), laneB(lane_ * (int)sizeof(T)),
           rec(buf_rsrc(a_.rec + (size_t)a_.step0[w] * K * 64, (long)ns * K * 1024)),
-          xq(buf_rsrc(a_.xq + r * a_.nslots + (size_t)a_.step0[w] * 64, (long)ns * 512)),
+          xq(buf_rsrc(a_.xq + r * a_.nslots + (size_t)a_.step0[w] * 64,
+                      (long)ns * kStep)),
           br(buf_rsrc(a_.brec + r * a_.nslots + (size_t)a_.step0[w] * 64,
-                      (long)ns * 512)),
+                      (long)ns * kStep)),
           xring(xring_), fifo(fifo_), sh(sh_) {}
 
     // the lane's FIFO slot of step s; read, then cleared for step s + kQ
-    __device__ __forceinline__ double take_fifo(int s) {
-        double *p = fifo + (s & (kQ - 1)) * 64 + lane;
-        const double v = *p;
-        *p = 0.0;
+    __device__ __forceinline__ T take_fifo(int s) {
+        T *p = fifo + (s & (kQ - 1)) * 64 + lane;
+        const T v = *p;
+        *p = T(0);
         return v;
     }
 
@@ -289,8 +363,7 @@ struct Consumer {
                                                                  tc * K * 1024, 0));
             }
         }
-        bb[B] = __builtin_bit_cast(
-                double, __builtin_amdgcn_raw_buffer_load_b64(br, lane8, tc * 512, 0));
+        bb[B] = FV<T>::load(br, laneB, tc * kStep);
     }
 
     // records of steps 0 .. D-1, oldest first.  Each is followed by a store
@@ -302,8 +375,7 @@ struct Consumer {
     __device__ __forceinline__ void prime() {
         if constexpr (U < D) {
             load_rec(U, U);
-            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, 0.0),
-                                                  buf_rsrc(a.xq, 0), lane8, 0, 0);
+            FV<T>::template store<0>(T(0), buf_rsrc(a.xq, 0), laneB, 0);
             __builtin_amdgcn_sched_barrier(0);
             prime<U + 1>();
         }
@@ -347,7 +419,7 @@ struct Consumer {
                 f[4 * k + 2] = rb[U][k].z;
                 f[4 * k + 3] = rb[U][k].w;
             }
-            double bv = bb[U];
+            T bv = bb[U];
             if constexpr (U == 0 && BLK) {
                 // every step of the block must be ready; then the FIFO
                 // values of its first kFP steps are read
@@ -363,7 +435,7 @@ struct Consumer {
             // With BLK every step of the block is ready once the block has
             // started, so a step's FIFO value is read kFP steps early and
             // the LDS latency leaves the chain.
-            double fv;
+            T fv;
             if constexpr (BLK) {
                 fv = fq[U];
                 if constexpr (U + kFP < NB) fq[U + kFP] = take_fifo(s + kFP);
@@ -378,33 +450,32 @@ struct Consumer {
             if constexpr (BLK) bv -= fv;
 #pragma unroll
             for (int j = 0; j < NRING; j++) {
-                const double xv =
-                    *(const double *)((const char *)xring + f[F::kSrc + j]);
-                bv = __builtin_fma(dbl(f[F::kVal + 2 * j], f[F::kVal + 2 * j + 1]),
-                                   xv, bv);
+                const T xv = *(const T *)((const char *)xring +
+                                          FV<T>::ring_off(f[F::kSrc + j]));
+                bv = FV<T>::fma(FV<T>::coef(f[F::kVal + 2 * j], f[F::kVal + 2 * j + 1]),
+                                xv, bv);
             }
             // the chain that carries the previous step's x: one FMA per
             // neighbour (the first needs no DPP)
-            double acc = bv;
+            T acc = bv;
             if constexpr (NDPP == 3) {
-                const double xl = dpp_from_lower(xprev);
-                const double xr = dpp_from_upper(xprev);
-                acc = __builtin_fma(dbl(f[0], f[1]), xprev, acc);
-                acc = __builtin_fma(dbl(f[2], f[3]), xl, acc);
-                acc = __builtin_fma(dbl(f[4], f[5]), xr, acc);
+                const T xl = FV<T>::from_lower(xprev);
+                const T xr = FV<T>::from_upper(xprev);
+                acc = FV<T>::fma(FV<T>::coef(f[0], f[1]), xprev, acc);
+                acc = FV<T>::fma(FV<T>::coef(f[2], f[3]), xl, acc);
+                acc = FV<T>::fma(FV<T>::coef(f[4], f[5]), xr, acc);
             } else if constexpr (NDPP == 1 || NDPP == 2) {
-                const double xn = NDPP == 1 ? dpp_from_lower(xprev) : dpp_from_upper(xprev);
-                acc = __builtin_fma(dbl(f[0], f[1]), xprev, acc);
-                acc = __builtin_fma(dbl(f[2], f[3]), xn, acc);
+                const T xn = NDPP == 1 ? FV<T>::from_lower(xprev) : FV<T>::from_upper(xprev);
+                acc = FV<T>::fma(FV<T>::coef(f[0], f[1]), xprev, acc);
+                acc = FV<T>::fma(FV<T>::coef(f[2], f[3]), xn, acc);
             }
-            const double xv = BLK ? acc : acc - fv;
+            const T xv = BLK ? acc : acc - fv;
             xprev = xv;
             if constexpr (NRING > 0) xring[(s & (kR - 1)) * 64 + lane] = xv;
             // every slot goes out (the epilogue reads them); device-coherent
             // (as an agent-scope store) so that published slots are visible
             // to the other waves' resolvers
-            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, xv), xq,
-                                                  lane8, s * 512, kSC1);
+            FV<T>::template store<kSC1>(xv, xq, laneB, s * kStep);
             if constexpr (U == NB - 1) {
                 // the block's FIFO slots are cleared: the resolver may
                 // refill them
@@ -421,10 +492,10 @@ struct Consumer {
         for (int sb = 0; sb < ns && ok; sb += NB) step<0>(sb);
         if (lane == 0) lds_store(&sh.cons, 1 << 30);
         if (a.trace && lane == 0 && r == 0) {
-            unsigned long long *T = a.trace + (size_t)w * 16;
-            T[1] = t_wait;
-            T[2] = now() - t0;
-            T[3] = ns;
+            unsigned long long *tr = a.trace + (size_t)w * 16;
+            tr[1] = t_wait;
+            tr[2] = now() - t0;
+            tr[3] = ns;
         }
     }
 };
@@ -446,10 +517,11 @@ __device__ inline unsigned long long rotr64(unsigned long long x, int r) {
 // no pause at batch boundaries.  An entry whose step the FIFO cannot hold
 // yet (kQ steps beyond the consumer) waits like a pending one.  A group
 // longer than the window is summed across rounds (`carry`).
-__device__ __forceinline__ void resolve(const FusedArgs &a, int w, int r,
-                                        int lane, double *fifo, double *rtmp,
-                                        int *gtmp, double *ctmp, Shared &sh) {
-    const double *xq = a.xq + r * a.nslots;
+template <class T>
+__device__ __forceinline__ void resolve(const FusedArgsT<T> &a, int w, int r,
+                                        int lane, T *fifo, T *rtmp, int *gtmp,
+                                        T *ctmp, Shared &sh) {
+    const T *xq = a.xq + r * a.nslots;
     const int e0 = a.gent0[w], e1 = a.gent0[w + 1];
     constexpr int kAll = 1 << 30;
     // steps before the first group need nothing
@@ -461,8 +533,7 @@ __device__ __forceinline__ void resolve(const FusedArgs &a, int w, int r,
     // per lane in flight: done one slot at a time this delayed the first
     // polls by ~10 us on the 1500^2 Laplacian
     if (a.xq_next) {
-        unsigned long long *nxt =
-            (unsigned long long *)(a.xq_next + r * a.nslots);
+        typename FV<T>::U *nxt = (typename FV<T>::U *)(a.xq_next + r * a.nslots);
         const int p0 = a.pub0[w], p1 = a.pub0[w + 1];
         for (int i0 = p0; i0 < p1; i0 += 8 * 64) {
             int sl[8];
@@ -473,14 +544,14 @@ __device__ __forceinline__ void resolve(const FusedArgs &a, int w, int r,
             }
 #pragma unroll
             for (int j = 0; j < 8; j++)
-                if (sl[j] >= 0) nxt[sl[j]] = kPending;
+                if (sl[j] >= 0) nxt[sl[j]] = FV<T>::kPend;
         }
     }
     unsigned long long t_poll = 0, nb = 0, nround = 0;
     const unsigned long long t0 = a.trace ? now() : 0;
     int e = e0 + lane;  // this lane's entry
     int src = 0, gm = 0, src2 = 0, gm2 = 0;
-    double gv = 0.0, gv2 = 0.0;  // metadata of e and of e + 64
+    T gv = 0, gv2 = 0;  // metadata of e and of e + 64
     if (e < e1) {
         src = a.gsrc[e];
         gv = a.gval[e];
@@ -491,16 +562,16 @@ __device__ __forceinline__ void resolve(const FusedArgs &a, int w, int r,
         gv2 = a.gval[e + 64];
         gm2 = a.gmeta[e + 64];
     }
-    double xv = e < e1 ? ld_x(&xq[src]) : 0.0;
+    T xv = e < e1 ? FV<T>::ld(&xq[src]) : T(0);
     int base = e0;
-    double carry = 0.0;
+    T carry = 0;
     int carry_g = -1;  // key (step << 6 | lane) of the group carried over
     unsigned spins = 0;
     while (base < e1) {
         nround++;
         const int lim = lds_load_u(&sh.cons) + kQ;
         const bool live = e < e1;
-        const bool pend = live && pending(xv);
+        const bool pend = live && FV<T>::pend(xv);
         const int key = gm >> 1, stp = gm >> kStepShift;
         const bool hold = live && (pend || stp >= lim);
         const int rot = (base - e0) & 63;   // lane of entry `base`
@@ -521,13 +592,13 @@ __device__ __forceinline__ void resolve(const FusedArgs &a, int w, int r,
         if (hi > 0) {
             nb++;
             spins = 0;
-            rtmp[lane] = live && !pend ? gv * xv : 0.0;
+            rtmp[lane] = live && !pend ? gv * xv : T(0);
             gtmp[lane] = live ? gm : -2;
             const int kprev = __shfl(key, (lane + 63) & 63, 64);
             // group heads sum their entries in order; a group begun in an
             // earlier round starts from the carried sum
             if (pos < hi && (pos == 0 || kprev != key)) {
-                double sum = (pos == 0 && carry_g == key) ? carry : 0.0;
+                T sum = (pos == 0 && carry_g == key) ? carry : T(0);
                 int q = pos, l = lane;
                 while (true) {
                     sum += rtmp[l];
@@ -576,7 +647,7 @@ __device__ __forceinline__ void resolve(const FusedArgs &a, int w, int r,
             __builtin_amdgcn_s_sleep(1);
         }
         // new entries and pending ones poll; then the next metadata
-        if ((adv && e < e1) || (!adv && pend)) xv = ld_x(&xq[src]);
+        if ((adv && e < e1) || (!adv && pend)) xv = FV<T>::ld(&xq[src]);
         if (adv && e + 64 < e1) {
             src2 = a.gsrc[e + 64];
             gv2 = a.gval[e + 64];
@@ -586,11 +657,11 @@ __device__ __forceinline__ void resolve(const FusedArgs &a, int w, int r,
     }
     if (a.trace) t_poll = now() - t0;
     if (a.trace && lane == 0 && r == 0) {
-        unsigned long long *T = a.trace + (size_t)w * 16;
-        T[7] = 0;
-        T[8] = t_poll;
-        T[9] = nb;
-        T[10] = nround;
+        unsigned long long *tr = a.trace + (size_t)w * 16;
+        tr[7] = 0;
+        tr[8] = t_poll;
+        tr[9] = nb;
+        tr[10] = nround;
     }
 }
 
@@ -606,33 +677,34 @@ constexpr int occupancy_target(bool blk, bool multi) {
 
 // One workgroup per (consumer wave w, right-hand side r): blockIdx.x =
 // r * W + w.  Right-hand sides share the records and nothing else.
-template <int NDPP, int NRING, bool BLK, bool MULTI>
+template <int NDPP, int NRING, bool BLK, bool MULTI, class T>
 __global__ void __launch_bounds__(kWG)
     __attribute__((amdgpu_waves_per_eu(occupancy_target(BLK, MULTI))))
-    k_fused(FusedArgs a) {
-    __shared__ double xring[NRING > 0 ? kR * 64 + 1 : 1];
-    __shared__ double fifo[kQ * 64];
-    __shared__ double rtmp[64];
+    k_fused(FusedArgsT<T> a) {
+    __shared__ T xring[NRING > 0 ? kR * 64 + 1 : 1];
+    __shared__ T fifo[kQ * 64];
+    __shared__ T rtmp[64];
     __shared__ int gtmp[64];
-    __shared__ double ctmp;
+    __shared__ T ctmp;
     __shared__ Shared sh;
 
+    if (a.left && *a.left == 0) return;  // every workgroup alike
     const int w = blockIdx.x % a.W, r = blockIdx.x / a.W;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    for (int i = threadIdx.x; i < kQ * 64; i += kWG) fifo[i] = 0.0;
+    for (int i = threadIdx.x; i < kQ * 64; i += kWG) fifo[i] = T(0);
     if (threadIdx.x == 0) {
         sh.ready = 0;
         sh.cons = 0;
         sh.abort = 0;
-        if (NRING > 0) xring[kR * 64] = 0.0;
+        if constexpr (NRING > 0) xring[kR * 64] = T(0);
     }
     __syncthreads();
     if (wave == 0) {
-        Consumer<NDPP, NRING, BLK> c(a, w, r, lane, xring, fifo, sh);
+        Consumer<NDPP, NRING, BLK, T> c(a, w, r, lane, xring, fifo, sh);
         c.run(w, r);
     } else if (wave == 1) {
-        resolve(a, w, r, lane, fifo, rtmp, gtmp, &ctmp, sh);
+        resolve<T>(a, w, r, lane, fifo, rtmp, gtmp, &ctmp, sh);
     }
 }
 
@@ -648,25 +720,29 @@ constexpr int kTile = 1024, kTileThreads = 256;
 // tile offsets sort in the low 12 bits of a key and fit an unsigned short
 static_assert(kTile <= 4096 && (kTile & (kTile - 1)) == 0, "kTile");
 
+// (left: as for k_fused, nothing to do when *left == 0)
+template <class T>
 __global__ void __launch_bounds__(kTileThreads)
     k_fused_prologue(const int *__restrict__ trow,
                      const unsigned short *__restrict__ tperm,
-                     const double *__restrict__ tinvd, long nslots,
-                     const double *__restrict__ b, long ldb, double *brec) {
-    __shared__ double buf[kTile];
+                     const T *__restrict__ tinvd, long nslots,
+                     const T *__restrict__ b, long ldb, T *brec,
+                     const int *__restrict__ left) {
+    if (left && *left == 0) return;
+    __shared__ T buf[kTile];
     const long base = (long)blockIdx.x * kTile;
     const int cnt = (int)min((long)kTile, nslots - base);
-    const double *br = b + blockIdx.y * ldb;
+    const T *br = b + blockIdx.y * ldb;
 #pragma unroll
     for (int k = 0; k < kTile / kTileThreads; k++) {
         const int i = k * kTileThreads + threadIdx.x;
         if (i < cnt) {
             const int row = trow[base + i];
-            buf[tperm[base + i]] = row >= 0 ? br[row] * tinvd[base + i] : 0.0;
+            buf[tperm[base + i]] = row >= 0 ? br[row] * tinvd[base + i] : T(0);
         }
     }
     __syncthreads();
-    double *out = brec + blockIdx.y * nslots + base;
+    T *out = brec + blockIdx.y * nslots + base;
 #pragma unroll
     for (int k = 0; k < kTile / kTileThreads; k++) {
         const int i = k * kTileThreads + threadIdx.x;
@@ -674,24 +750,26 @@ __global__ void __launch_bounds__(kTileThreads)
     }
 }
 
+template <class T>
 __global__ void __launch_bounds__(kTileThreads)
     k_fused_epilogue(const int *__restrict__ trow,
                      const unsigned short *__restrict__ tperm, long nslots,
-                     const double *__restrict__ xq, double *x, long ldx,
-                     const int *status) {
-    __shared__ double buf[kTile];
+                     const T *__restrict__ xq, T *x, long ldx,
+                     const int *status, const int *__restrict__ left) {
+    if (left && *left == 0) return;
+    __shared__ T buf[kTile];
     const long base = (long)blockIdx.x * kTile;
     const int cnt = (int)min((long)kTile, nslots - base);
-    const double *in = xq + blockIdx.y * nslots + base;
+    const T *in = xq + blockIdx.y * nslots + base;
     // after a watchdog trip every x of the plan's solves is a quiet NaN
     const bool tripped = (*status & 7) != 0;
 #pragma unroll
     for (int k = 0; k < kTile / kTileThreads; k++) {
         const int i = k * kTileThreads + threadIdx.x;
-        if (i < cnt) buf[i] = tripped ? __builtin_nan("") : in[i];
+        if (i < cnt) buf[i] = tripped ? T(__builtin_nan("")) : in[i];
     }
     __syncthreads();
-    double *xr = x + blockIdx.y * ldx;
+    T *xr = x + blockIdx.y * ldx;
 #pragma unroll
     for (int k = 0; k < kTile / kTileThreads; k++) {
         const int i = k * kTileThreads + threadIdx.x;
@@ -702,9 +780,17 @@ __global__ void __launch_bounds__(kTileThreads)
     }
 }
 
-__global__ void k_fill_pending(double *p, long n) {
+template <class T>
+__global__ void k_fill_pending(T *p, long n, const int *__restrict__ left) {
+    if (left && *left == 0) return;
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) ((unsigned long long *)p)[i] = kPending;
+    if (i < n) ((typename FV<T>::U *)p)[i] = FV<T>::kPend;
+}
+
+// FP32 copies of the fused schedule's global coefficients and 1/diag
+__global__ void k_to32(const double *__restrict__ in, float *out, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (float)in[i];
 }
 
 }  // namespace
@@ -730,6 +816,13 @@ struct FusedPlan {
     int nr_alloc = 0;
     std::vector<char> armed[2];
     double *d_brec = nullptr;  // b/diag in slot order, nr_alloc right-hand sides
+    // FP32 solves (the refinement's inner solver), built on first use:
+    // global entries' values and 1/diag in FP32 (the step records are
+    // shared with FP64); slot-order x and b/diag for nr32 right-hand sides
+    // (x refilled with the sentinel before every launch)
+    float *d_gval32 = nullptr, *d_tinvd32 = nullptr;
+    float *d_xq32 = nullptr, *d_brec32 = nullptr;
+    int nr32 = 0;
     int ncu = 0, per_cu = 0;   // CUs; k_fused workgroups resident per CU
     bool small = false;        // single-column solves are ordinary launches
     unsigned long long *d_trace = nullptr;  // RB_FUSED_TRACE=1
@@ -1348,28 +1441,29 @@ int fused_max_waves() {
     return ncu > 0 ? ncu : 256;
 }
 
-template <int NDPP, int NRING>
+template <int NDPP, int NRING, class T>
 static const void *kernel_of(bool blk, bool multi) {
-    if (!blk) return (const void *)k_fused<NDPP, NRING, false, false>;
-    return multi ? (const void *)k_fused<NDPP, NRING, true, true>
-                 : (const void *)k_fused<NDPP, NRING, true, false>;
+    if (!blk) return (const void *)k_fused<NDPP, NRING, false, false, T>;
+    return multi ? (const void *)k_fused<NDPP, NRING, true, true, T>
+                 : (const void *)k_fused<NDPP, NRING, true, false, T>;
 }
 
 // the kernel for F's record format; multi: a launch of several columns
+template <class T = double>
 static const void *pick_kernel(const FusedPlan *F, bool multi) {
     const int r = F->NRING;
     const bool b = F->blk;
     if (F->NDPP == 3)
-        return r == 0 ? kernel_of<3, 0>(b, multi) : r == 2 ? kernel_of<3, 2>(b, multi)
-             : r == 4 ? kernel_of<3, 4>(b, multi) : kernel_of<3, 8>(b, multi);
+        return r == 0 ? kernel_of<3, 0, T>(b, multi) : r == 2 ? kernel_of<3, 2, T>(b, multi)
+             : r == 4 ? kernel_of<3, 4, T>(b, multi) : kernel_of<3, 8, T>(b, multi);
     if (F->NDPP == 1)
-        return r == 0 ? kernel_of<1, 0>(b, multi) : r == 2 ? kernel_of<1, 2>(b, multi)
-             : r == 4 ? kernel_of<1, 4>(b, multi) : kernel_of<1, 8>(b, multi);
+        return r == 0 ? kernel_of<1, 0, T>(b, multi) : r == 2 ? kernel_of<1, 2, T>(b, multi)
+             : r == 4 ? kernel_of<1, 4, T>(b, multi) : kernel_of<1, 8, T>(b, multi);
     if (F->NDPP == 2)
-        return r == 0 ? kernel_of<2, 0>(b, multi) : r == 2 ? kernel_of<2, 2>(b, multi)
-             : r == 4 ? kernel_of<2, 4>(b, multi) : kernel_of<2, 8>(b, multi);
-    return r == 0 ? kernel_of<0, 0>(b, multi) : r == 2 ? kernel_of<0, 2>(b, multi)
-         : r == 4 ? kernel_of<0, 4>(b, multi) : kernel_of<0, 8>(b, multi);
+        return r == 0 ? kernel_of<2, 0, T>(b, multi) : r == 2 ? kernel_of<2, 2, T>(b, multi)
+             : r == 4 ? kernel_of<2, 4, T>(b, multi) : kernel_of<2, 8, T>(b, multi);
+    return r == 0 ? kernel_of<0, 0, T>(b, multi) : r == 2 ? kernel_of<0, 2, T>(b, multi)
+         : r == 4 ? kernel_of<0, 4, T>(b, multi) : kernel_of<0, 8, T>(b, multi);
 }
 
 // Slot-order x buffers for nr right-hand sides (grown on demand; a fresh
@@ -1390,7 +1484,7 @@ static rb_status ensure_rhs(FusedPlan *F, int nr, hipStream_t s) {
         return RB_ERR_NO_MEMORY;
     }
     const long n = 2L * nr * (long)nslots;
-    k_fill_pending<<<(int)((n + 255) / 256), 256, 0, s>>>(F->d_xq, n);
+    k_fill_pending<double><<<(int)((n + 255) / 256), 256, 0, s>>>(F->d_xq, n, nullptr);
     RB_HIP(hipGetLastError());
     F->nr_alloc = nr;
     F->armed[0].assign(nr, 1);
@@ -1431,8 +1525,9 @@ static void device_shape(FusedPlan *F) {
 
 FusedPlan::~FusedPlan() {
     if (small) g_small_plans.fetch_sub(1);
-    if (d_xq) (void)hipFree(d_xq);
-    if (d_brec) (void)hipFree(d_brec);
+    for (void *q : {(void *)d_xq, (void *)d_brec, (void *)d_gval32,
+                    (void *)d_tinvd32, (void *)d_xq32, (void *)d_brec32})
+        if (q) (void)hipFree(q);
 }
 
 // Right-hand sides that fit one cooperative launch (W workgroups each).
@@ -1450,23 +1545,34 @@ static int rhs_per_launch(FusedPlan *F) {
 // next solve, whose published slots the launch re-arms (may be null); brec:
 // nr slot-order b/diag buffers.  Buffers of one right-hand side are nslots
 // apart.
-static rb_status launch_group(rb_plan *p, const double *b, int ldb, double *x,
-                              int ldx, int nr, double *cur, double *nxt,
-                              double *brec, hipStream_t s) {
+// T: FP64, or FP32 on the FP32 tables (left: see k_fused).
+template <class T>
+static rb_status launch_group(rb_plan *p, const T *b, int ldb, T *x, int ldx,
+                              int nr, T *cur, T *nxt, T *brec, hipStream_t s,
+                              const int *left = nullptr) {
     FusedPlan *F = p->fused;
+    constexpr bool f32 = sizeof(T) == 4;
     const size_t nslots = (size_t)F->nsteps * 64;
     const dim3 tiles((unsigned)((nslots + kTile - 1) / kTile), (unsigned)nr);
-    k_fused_prologue<<<tiles, kTileThreads, 0, s>>>(
-        F->d_trow, F->d_tperm, F->d_tinvd, (long)nslots, b, ldb, brec);
+    const T *tinvd, *gval;
+    if constexpr (f32) {
+        tinvd = F->d_tinvd32;
+        gval = F->d_gval32;
+    } else {
+        tinvd = F->d_tinvd;
+        gval = F->d_gval;
+    }
+    k_fused_prologue<T><<<tiles, kTileThreads, 0, s>>>(
+        F->d_trow, F->d_tperm, tinvd, (long)nslots, b, ldb, brec, left);
     RB_HIP(hipGetLastError());
-    FusedArgs args{F->d_step0, F->d_gent0, F->d_pub0, F->d_rec,
-                   F->d_gsrc, F->d_gval, F->d_gmeta, F->d_pub, cur, nxt,
-                   brec, (long)nslots, F->W, p->d_status,
-                   p->spin_limit, F->d_trace};
+    FusedArgsT<T> args{F->d_step0, F->d_gent0, F->d_pub0, F->d_rec,
+                       F->d_gsrc, gval, F->d_gmeta, F->d_pub, cur, nxt,
+                       brec, (long)nslots, F->W, p->d_status,
+                       p->spin_limit, F->d_trace, left};
     void *kargs[] = {(void *)&args};
     mark_start(p, s);
     const bool ordinary = F->W * nr <= small_grid(F->ncu);
-    const void *kern = pick_kernel(F, nr > 1);
+    const void *kern = pick_kernel<T>(F, nr > 1);
     hipError_t e =
         ordinary
             ? hipLaunchKernel(kern, dim3(F->W * nr), dim3(kWG), kargs, 0, s)
@@ -1476,8 +1582,8 @@ static rb_status launch_group(rb_plan *p, const double *b, int ldb, double *x,
         return status_of_hip(e, ordinary ? "hipLaunchKernel(k_fused)"
                                          : "hipLaunchCooperativeKernel(k_fused)");
     mark_end(p, s);
-    k_fused_epilogue<<<tiles, kTileThreads, 0, s>>>(
-        F->d_trow, F->d_tperm, (long)nslots, cur, x, ldx, p->d_status);
+    k_fused_epilogue<T><<<tiles, kTileThreads, 0, s>>>(
+        F->d_trow, F->d_tperm, (long)nslots, cur, x, ldx, p->d_status, left);
     RB_HIP(hipGetLastError());
     return RB_OK;
 }
@@ -1502,13 +1608,13 @@ rb_status solve_fused(rb_plan *p, const double *d_b, int ldb, double *d_x,
         // regions this launch polls must hold the sentinel
         for (int r = 0; r < nr; r++)
             if (!F->armed[cb][r]) {
-                k_fill_pending<<<(int)((nslots + 255) / 256), 256, 0, s>>>(
-                    cur + r * nslots, (long)nslots);
+                k_fill_pending<double><<<(int)((nslots + 255) / 256), 256, 0, s>>>(
+                    cur + r * nslots, (long)nslots, nullptr);
                 RB_HIP(hipGetLastError());
             }
-        RB_TRY(launch_group(p, d_b + (size_t)k0 * ldb, ldb,
-                            d_x + (size_t)k0 * ldx, ldx, nr, cur, nxt,
-                            F->d_brec, s));
+        RB_TRY(launch_group<double>(p, d_b + (size_t)k0 * ldb, ldb,
+                                    d_x + (size_t)k0 * ldx, ldx, nr, cur, nxt,
+                                    F->d_brec, s));
         for (int r = 0; r < nr; r++) {
             F->armed[cb][r] = 0;
             F->armed[nb][r] = 1;
@@ -1547,11 +1653,73 @@ rb_status solve_fused_ws(rb_plan *p, const double *d_b, int ldb, double *d_x,
         // the workspace keeps whatever the last call left: every slot of
         // every column goes back to the sentinel, and nothing is re-armed
         const long n = (long)nr * (long)nslots;
-        k_fill_pending<<<(int)((n + 255) / 256), 256, 0, s>>>(cur, n);
+        k_fill_pending<double><<<(int)((n + 255) / 256), 256, 0, s>>>(cur, n, nullptr);
         RB_HIP(hipGetLastError());
-        RB_TRY(launch_group(p, d_b + (size_t)k0 * ldb, ldb,
-                            d_x + (size_t)k0 * ldx, ldx, nr, cur, nullptr,
-                            brec, s));
+        RB_TRY(launch_group<double>(p, d_b + (size_t)k0 * ldb, ldb,
+                                    d_x + (size_t)k0 * ldx, ldx, nr, cur, nullptr,
+                                    brec, s));
+    }
+    return RB_OK;
+}
+
+// FP32 tables (once) and buffers for nr right-hand sides.
+static rb_status ensure32(rb_plan *p, int nr, hipStream_t s) {
+    FusedPlan *F = p->fused;
+    const size_t nslots = (size_t)F->nsteps * 64;
+    if (!F->d_gval32) {
+        const size_t nent = std::max(1, F->nent);
+        if (hipMalloc(&F->d_gval32, sizeof(float) * nent) != hipSuccess ||
+            hipMalloc(&F->d_tinvd32, sizeof(float) * nslots) != hipSuccess) {
+            (void)hipGetLastError();
+            if (F->d_gval32) (void)hipFree(F->d_gval32);
+            F->d_gval32 = nullptr;  // built on the next call
+            set_error(RB_ERR_NO_MEMORY, "hipMalloc failed (FP32 fused tables)");
+            return RB_ERR_NO_MEMORY;
+        }
+        k_to32<<<(int)((nent + 255) / 256), 256, 0, s>>>(F->d_gval, F->d_gval32,
+                                                        (long)nent);
+        k_to32<<<(int)((nslots + 255) / 256), 256, 0, s>>>(F->d_tinvd, F->d_tinvd32,
+                                                          (long)nslots);
+        RB_HIP(hipGetLastError());
+    }
+    if (nr > F->nr32) {
+        RB_HIP(hipStreamSynchronize(s));
+        if (F->d_xq32) RB_HIP(hipFree(F->d_xq32));
+        if (F->d_brec32) RB_HIP(hipFree(F->d_brec32));
+        F->d_xq32 = F->d_brec32 = nullptr;
+        F->nr32 = 0;
+        if (hipMalloc(&F->d_xq32, sizeof(float) * nr * nslots) != hipSuccess ||
+            hipMalloc(&F->d_brec32, sizeof(float) * nr * nslots) != hipSuccess) {
+            (void)hipGetLastError();
+            set_error(RB_ERR_NO_MEMORY, "hipMalloc failed (FP32 slot-order buffers)");
+            return RB_ERR_NO_MEMORY;
+        }
+        F->nr32 = nr;
+    }
+    return RB_OK;
+}
+
+rb_status solve_fused32(rb_plan *p, const float *d_b, int ldb, float *d_x,
+                        int ldx, int nrhs, hipStream_t s, const int *left) {
+    FusedPlan *F = p->fused;
+    if (!F) {
+        set_error(RB_ERR_INTERNAL, "fused schedule not built");
+        return RB_ERR_INTERNAL;
+    }
+    if (p->m == 0) return RB_OK;
+    const size_t nslots = (size_t)F->nsteps * 64;
+    device_shape(F);
+    const int cap = nrhs > 1 ? rhs_per_launch(F) : 1;
+    RB_TRY(ensure32(p, std::min(nrhs, cap), s));
+    for (int k0 = 0; k0 < nrhs; k0 += cap) {
+        const int nr = std::min(cap, nrhs - k0);
+        // every slot back to the sentinel: nothing is re-armed
+        const long n = (long)nr * (long)nslots;
+        k_fill_pending<float><<<(int)((n + 255) / 256), 256, 0, s>>>(F->d_xq32, n, left);
+        RB_HIP(hipGetLastError());
+        RB_TRY(launch_group<float>(p, d_b + (size_t)k0 * ldb, ldb,
+                                   d_x + (size_t)k0 * ldx, ldx, nr, F->d_xq32,
+                                   nullptr, F->d_brec32, s, left));
     }
     return RB_OK;
 }

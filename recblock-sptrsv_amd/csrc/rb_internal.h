@@ -248,6 +248,11 @@ rb_status solve_blocked_ws(rb_plan *p, const double *d_b, int ldb,
 rb_status solve_fused_ws(rb_plan *p, const double *d_b, int ldb, double *d_x,
                          int ldx, int nrhs, hipStream_t s, double *ws,
                          size_t ws_bytes);
+// FP32 solve on the fused schedule (FP32 copies of its tables, made on
+// first use): the refinement's inner solver; does nothing once *left == 0
+// (left may be null)
+rb_status solve_fused32(rb_plan *p, const float *d_b, int ldb, float *d_x,
+                        int ldx, int nrhs, hipStream_t s, const int *left);
 // events around a solve's main launches when timing is on: from the start
 // of the first (the entry point sets mark_first) to the end of the last
 inline void mark_start(rb_plan *p, hipStream_t s) {

@@ -510,10 +510,13 @@ def test_fused_many_rhs_match_single(oracle, torch, name, make):
     ("banded20000", lambda: gen.banded(20000, seed=5)),
     ("powerlaw30000", lambda: gen.powerlaw(30000, seed=6)),
 ])
-def test_f32_refinement_reaches_fp64_accuracy(oracle, torch, name, make):
+@pytest.mark.parametrize("schedule", [0, 1, 2])
+def test_f32_refinement_reaches_fp64_accuracy(oracle, torch, name, make, schedule):
+    """schedule 2: the FP32 corrections run on the fused dataflow kernel
+    (FP32 copies of its records); 1: on the blocked walk."""
     f = make()
-    _say("f32ir", name)
-    p = _plan(f, 2, mixed=True)
+    _say("f32ir", name, schedule)
+    p = _plan(f, 2, mixed=True, schedule=schedule)
     b = gen.rhs(f.n, seed=4)
     x, it, rel = p.solve_f32ir(torch.as_tensor(b, device="cuda"), tol=1e-12)
     x = x.cpu().numpy()
@@ -552,10 +555,11 @@ def test_f32_refinement_reaches_fp64_accuracy(oracle, torch, name, make):
     p.check()                                        # cleared
 
 
-def test_f32_refinement_many_rhs_and_errors(torch):
+@pytest.mark.parametrize("schedule,k", [(0, 3), (1, 3), (2, 3), (2, 40)])
+def test_f32_refinement_many_rhs_and_errors(torch, schedule, k):
     f = gen.laplacian(50, 40)
-    p = _plan(f, 2, mixed=True)
-    k = 3
+    _say("f32ir many", schedule, k)
+    p = _plan(f, 2, mixed=True, schedule=schedule)
     B = np.random.default_rng(3).uniform(-1, 1, (k, f.n))
     B[1] = 0.0                                       # zero column: x = 0
     Bd = torch.as_tensor(B, device="cuda").t()
@@ -563,8 +567,11 @@ def test_f32_refinement_many_rhs_and_errors(torch):
     X = X.t().cpu().numpy()
     assert rel <= 1e-12
     assert np.all(X[1] == 0.0)
-    for j in (0, 2):
+    for j in [0] + list(range(2, k)):
         assert residual(f, X[j], B[j]) < RES_TOL
+    # the same columns again: the FP32 slot buffers start over each solve
+    X2 = p.solve_f32ir(Bd)[0].t().cpu().numpy()
+    assert np.array_equal(X2, X)
     q = _plan(f, 2)                                  # no FP32 copies
     from rbsptrsv import RbError
     with pytest.raises(RbError) as ei:
