@@ -242,9 +242,11 @@ rb_status rb_plan_trace(const rb_plan *plan, unsigned long long *counters,
 
 /* FP32 solves with FP64 iterative refinement (plan created with
  * rb_opts.mixed = 1).  Per right-hand side: x = 0, r = b; repeat
- * { d = solve((float) r) with the FP32 copies of the block values, in the
- * blocked schedule's kernels; x += d in FP64; r = b - A x in FP64 on the
- * device } until max_i |r_i| <= tol * max_i |b_i| or max_iter corrections.
+ * { d = solve((float) r) in FP32 on the plan's schedule (the blocked
+ * walk's kernels on FP32 copies of the block values, or the fused kernel's
+ * FP32 form, whose FP32 tables and buffers are allocated on the first
+ * call); x += d in FP64; r = b - A x in FP64 on the device } until
+ * max_i |r_i| <= tol * max_i |b_i| or max_iter corrections.
  * FP64 is used only for the residual and the update of x.  d_b / d_x as in
  * rb_solve.  The loop runs on the device, up to 32 columns at a time;
  * each column stops correcting once it has converged.  With iters or
