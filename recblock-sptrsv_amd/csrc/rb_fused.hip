@@ -689,7 +689,17 @@ __global__ void __launch_bounds__(kWG)
     __shared__ Shared sh;
 
     if (a.left && *a.left == 0) return;  // every workgroup alike
-    const int w = blockIdx.x % a.W, r = blockIdx.x / a.W;
+    int w = blockIdx.x % a.W;
+    const int r = blockIdx.x / a.W;
+    if (gridDim.x == (unsigned)a.W) {
+        // one column: workgroups go to the 8 XCDs round-robin (XCD =
+        // blockIdx mod 8), so XCD x takes the consecutive waves from
+        // x q + min(x, W mod 8), q = W / 8 (a bijection onto [0, W)): most
+        // wave-to-next-wave hand-offs stay in one XCD's L2.  Laplacian
+        // 1500^2, 24 waves: 216.2 -> 213.1 us.
+        const int b = blockIdx.x, x = b & 7, k = b >> 3, q = a.W >> 3, rem = a.W & 7;
+        w = x * q + min(x, rem) + k;
+    }
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     for (int i = threadIdx.x; i < kQ * 64; i += kWG) fifo[i] = T(0);
