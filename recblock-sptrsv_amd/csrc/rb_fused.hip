@@ -803,6 +803,91 @@ __global__ void k_to32(const double *__restrict__ in, float *out, long n) {
     if (i < n) out[i] = (float)in[i];
 }
 
+// The values of the fused tables, computed where the host left (entry e,
+// permuted row r) index pairs (e = -1: zero): record coefficients
+// -val[e] / diag[r] (val: the permuted CSR's values, invd: 1/diag in
+// permuted order), one thread per lane-step record of K words ...
+__global__ void k_fill_rec_values(int4 *rec, long nls, int K, int kSrc, int nring,
+                                  int kVal, const double *__restrict__ val,
+                                  const double *__restrict__ invd) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nls) return;
+    const long st = t / 64;
+    const int l = (int)(t % 64);
+    int f[32];
+    for (int k = 0; k < K; k++) {
+        const int4 w = rec[(st * K + k) * 64 + l];
+        f[4 * k] = w.x;
+        f[4 * k + 1] = w.y;
+        f[4 * k + 2] = w.z;
+        f[4 * k + 3] = w.w;
+    }
+    auto fill = [&](int i) {
+        const int e = f[i], r = f[i + 1];
+        const double v = e < 0 ? 0.0 : -val[e] * invd[r];
+        f[i] = __double2loint(v);
+        f[i + 1] = __double2hiint(v);
+    };
+    for (int i = 0; i < kSrc; i += 2) fill(i);
+    for (int j = 0; j < nring; j++) fill(kVal + 2 * j);
+    for (int k = 0; k < K; k++)
+        rec[(st * K + k) * 64 + l] = make_int4(f[4 * k], f[4 * k + 1], f[4 * k + 2],
+                                               f[4 * k + 3]);
+}
+
+// ... global entries' values val[e] / diag[r] ...
+__global__ void k_fill_gval(double *gval, long n, const double *__restrict__ val,
+                            const double *__restrict__ invd) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int2 er = reinterpret_cast<const int2 *>(gval)[i];
+    gval[i] = er.x < 0 ? 0.0 : val[er.x] * invd[er.y];
+}
+
+// The tile tables of the prologue and epilogue, one workgroup per tile of
+// kTile slots: trow holds each slot's permuted row (-1: a padding lane) on
+// entry; the tile's slots are sorted by (original row = perm[permuted row],
+// padding last, then slot) and trow / tperm / tinvd receive, in that
+// order, the original row, the slot's offset in the tile and 1/diag.
+// Every read of trow happens before the tile's first write.
+__global__ void __launch_bounds__(kTileThreads)
+    k_tiles(int *trow, const int *__restrict__ perm, const double *__restrict__ invd,
+            long nslots, unsigned short *tperm, double *tinvd) {
+    __shared__ unsigned long long key[kTile];
+    __shared__ int prow[kTile];
+    const long base = (long)blockIdx.x * kTile;
+    const int cnt = (int)min((long)kTile, nslots - base);
+    for (int i = threadIdx.x; i < kTile; i += kTileThreads) {
+        const int pr = i < cnt ? trow[base + i] : -1;
+        prow[i] = pr;
+        const unsigned long long o = pr >= 0 ? (unsigned)perm[pr] : 0xffffffffu;
+        key[i] = o << 12 | (unsigned)i;  // slots past cnt sort last
+    }
+    __syncthreads();
+    for (int k = 2; k <= kTile; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < kTile; i += kTileThreads) {
+                const int ij = i ^ j;
+                if (ij > i) {
+                    const unsigned long long x = key[i], y = key[ij];
+                    if ((x > y) == ((i & k) == 0)) {
+                        key[i] = y;
+                        key[ij] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    for (int i = threadIdx.x; i < cnt; i += kTileThreads) {
+        const unsigned long long kv = key[i];
+        const int q = (int)(kv & (kTile - 1));
+        const unsigned o = (unsigned)(kv >> 12);
+        tperm[base + i] = (unsigned short)q;
+        trow[base + i] = o == 0xffffffffu ? -1 : (int)o;
+        tinvd[base + i] = prow[q] < 0 ? 0.0 : invd[prow[q]];
+    }
+}
+
 }  // namespace
 
 // ----------------------------------------------------------------- host
@@ -908,10 +993,11 @@ template <class T>
 using HostVec = std::vector<T, NoInit<T>>;
 
 // Host partitioner and record builder.  rp/cc: permuted CSR structure
-// (ascending columns); lp: BFS level boundaries.  The values, the final
-// permutation and 1/diag are fetched from the device (d_cval, the plan's
-// levelItem and invd) only once the schedule is known to be worth
-// building.
+// (ascending columns); lp: BFS level boundaries.  The final permutation
+// (the plan's levelItem) is fetched only once the schedule is known to be
+// worth building; the values (d_cval and the plan's invd) never leave the
+// device: the uploaded tables carry index pairs in their place, which the
+// k_fill_* kernels turn into values.
 rb_status build_fused_host(rb_plan *p, const HostVec<int> &rp,
                            const HostVec<int> &cc, const HostVec<int> &lp,
                            const double *d_cval, int Wmax, hipStream_t s) {
@@ -1161,20 +1247,6 @@ rb_status build_fused_host(rb_plan *p, const HostVec<int> &rp,
         }
         return RB_OK;
     }
-    // the rest of the tables: values, permutation, 1/diag
-    HostVec<double> cv((size_t)std::max(1, p->nnz)), invd((size_t)std::max(1, m));
-    HostVec<int> lf((size_t)std::max(1, m));
-    if (p->nnz > 0)
-        RB_HIP(hipMemcpyAsync(cv.data(), d_cval, sizeof(double) * p->nnz,
-                              hipMemcpyDeviceToHost, s));
-    if (m > 0) {
-        RB_HIP(hipMemcpyAsync(lf.data(), p->d_levelItem, sizeof(int) * m,
-                              hipMemcpyDeviceToHost, s));
-        RB_HIP(hipMemcpyAsync(invd.data(), p->d_invd, sizeof(double) * m,
-                              hipMemcpyDeviceToHost, s));
-    }
-    RB_HIP(hipStreamSynchronize(s));
-    mark("values");
     const long n_lo = tot.lo, n_up = tot.up;
     const int NDPP = !(n_dpp > 0 && 4 * n_dpp >= n_all) ? 0
                    : n_up == 0 ? 1 : n_lo == 0 ? 2 : 3;
@@ -1204,26 +1276,25 @@ rb_status build_fused_host(rb_plan *p, const HostVec<int> &rp,
     // every element of these is written by the emitting threads: no
     // serial zero-fill of ~100 MB before they start
     std::vector<int4, NoInit<int4>> rec((size_t)NS * K * 64);
-    std::vector<int, NoInit<int>> origv((size_t)NS * 64);
-    std::vector<double, NoInit<double>> sinvd((size_t)NS * 64);
+    std::vector<int, NoInit<int>> srow((size_t)NS * 64);  // permuted row, -1
     const int NB = block_steps(K);
     // Per wave, in parallel on host threads (waves write disjoint record,
     // slot and 1/diag ranges; their global entries, sources and block
     // wait edges are kept per wave and joined in wave order below).
     struct WaveOut {
         std::vector<int> gsrc, gmeta, srcq;
-        std::vector<double> gval;
+        std::vector<int2> gval;  // (entry, row) of the value
         std::vector<std::pair<int, int>> edges;
     };
     std::vector<WaveOut> wout(W);
     auto emit_wave = [&](int w) {
         WaveOut &o = wout[w];
         std::vector<int> f(4 * std::max(K, 1));
-        auto put = [&](int at, double v) {
-            long long bits;
-            std::memcpy(&bits, &v, 8);
-            f[at] = (int)(bits & 0xffffffff);
-            f[at + 1] = (int)(bits >> 32);
+        // a coefficient's place holds (entry, permuted row) until the
+        // device computes -val[entry] / diag[row] there; entry -1: zero
+        auto put = [&](int at, int e, int r) {
+            f[at] = e;
+            f[at + 1] = r;
         };
         for (int si = 0; si < (int)steps[w].size(); si++) {
             const auto &st = steps[w][si];
@@ -1231,11 +1302,14 @@ rb_status build_fused_host(rb_plan *p, const HostVec<int> &rp,
             for (int l = 0; l < 64; l++) {
                 const int q = st[l];
                 std::fill(f.begin(), f.end(), 0);
-                for (int j = 0; j < NRING; j++) f[kSrc + j] = kZeroOff;
+                for (int i = 0; i < kSrc; i += 2) f[i] = -1;
+                for (int j = 0; j < NRING; j++) {
+                    f[kSrc + j] = kZeroOff;
+                    f[kVal + 2 * j] = -1;
+                }
                 bool hasg = false;
                 if (q >= 0) {
                     const int r = pof(q);
-                    origv[(size_t)(step0[w] + si) * 64 + l] = lf[r];
                     int nl = 0;
                     for (int e = rp[r]; e < rp[r + 1]; e++) {
                         if (cc[e] == r) continue;
@@ -1244,27 +1318,25 @@ rb_status build_fused_host(rb_plan *p, const HostVec<int> &rp,
                             const int du = lanew[u] - l;  // -1, 0, +1
                             // lane l at [0..1]; the one side of NDPP 1 / 2
                             // at [2..3]; NDPP 3: l-1 at [2..3], l+1 at [4..5]
-                            put(du == 0 ? 0 : (du < 0 || NDPP == 2) ? 2 : 4,
-                                -cv[e] * invd[r]);
+                            put(du == 0 ? 0 : (du < 0 || NDPP == 2) ? 2 : 4, e, r);
                         } else if (is_loc(q, u) && nl < NRING) {
                             f[kSrc + nl] = ((stepw[u] % kR) * 64 + lanew[u]) * 8;
-                            put(kVal + 2 * nl, -cv[e] * invd[r]);
+                            put(kVal + 2 * nl, e, r);
                             nl++;
                         } else {
                             o.srcq.push_back(u);
                             o.edges.emplace_back((step0[w] + si) / NB,
                                                  slot_of(u) / 64 / NB);
                             o.gsrc.push_back(slot_of(u));
-                            o.gval.push_back(cv[e] * invd[r]);
+                            o.gval.push_back(make_int2(e, r));
                             o.gmeta.push_back(si << kStepShift | l << 1);
                             hasg = true;
                         }
                     }
                     if (hasg) o.gmeta.back() |= 1;  // last of the row's group
-                    sinvd[(size_t)(step0[w] + si) * 64 + l] = invd[r];
+                    srow[(size_t)(step0[w] + si) * 64 + l] = r;
                 } else {  // padding lane
-                    origv[(size_t)(step0[w] + si) * 64 + l] = -1;
-                    sinvd[(size_t)(step0[w] + si) * 64 + l] = 0.0;
+                    srow[(size_t)(step0[w] + si) * 64 + l] = -1;
                 }
                 for (int k = 0; k < K; k++)
                     rec[sbase + (size_t)k * 64 + l] =
@@ -1283,42 +1355,8 @@ rb_status build_fused_host(rb_plan *p, const HostVec<int> &rp,
         for (auto &t : pool) t.join();
     }
     mark("emit");
-    // every tile's slots listed by ascending row, padding lanes last
-    const size_t nslot_all = (size_t)NS * 64;
-    const int ntile = (int)((nslot_all + kTile - 1) / kTile);
-    std::vector<int> trow(nslot_all);
-    std::vector<unsigned short> tperm(nslot_all);
-    std::vector<double> tinvd(nslot_all);
-    {
-        auto sort_tile = [&](int t) {
-            const size_t base = (size_t)t * kTile;
-            const int cnt = (int)std::min<size_t>(kTile, nslot_all - base);
-            std::vector<unsigned long long> key(cnt);
-            for (int i = 0; i < cnt; i++) {
-                const int o = origv[base + i];
-                const unsigned long long r = o >= 0 ? (unsigned)o : 0xffffffffu;
-                key[i] = r << 12 | (unsigned)i;
-            }
-            std::sort(key.begin(), key.end());
-            for (int i = 0; i < cnt; i++) {
-                const int q = (int)(key[i] & (kTile - 1));
-                tperm[base + i] = (unsigned short)q;
-                trow[base + i] = origv[base + q];
-                tinvd[base + i] = sinvd[base + q];
-            }
-        };
-        const int nth = std::max(1, std::min({ntile, 16, (int)std::thread::hardware_concurrency()}));
-        std::vector<std::thread> pool;
-        std::atomic<int> next{0};
-        for (int t = 0; t < nth; t++)
-            pool.emplace_back([&]() {
-                for (int k; (k = next.fetch_add(1)) < ntile;) sort_tile(k);
-            });
-        for (auto &t : pool) t.join();
-    }
-    mark("tiles");
     std::vector<int> gsrc, gmeta;
-    std::vector<double> gval;
+    std::vector<int2> gval;
     std::vector<char> needpub(m, 0);
     // (waiting block, producing block) of every global entry; waves start
     // at multiples of NB, so block = global step / NB
@@ -1350,7 +1388,7 @@ rb_status build_fused_host(rb_plan *p, const HostVec<int> &rp,
     pub0[W] = (int)pub.size();
     if (gsrc.empty()) {
         gsrc.push_back(0);
-        gval.push_back(0.0);
+        gval.push_back(make_int2(-1, 0));
         gmeta.push_back(1);
     }
     if (pub.empty()) pub.push_back(0);
@@ -1395,13 +1433,24 @@ rb_status build_fused_host(rb_plan *p, const HostVec<int> &rp,
     RB_TRY(up(F->d_gent0, gent0.data(), sizeof(int) * gent0.size()));
     RB_TRY(up(F->d_pub0, pub0.data(), sizeof(int) * pub0.size()));
     RB_TRY(up(F->d_rec, rec.data(), sizeof(int4) * rec.size()));
-    RB_TRY(up(F->d_trow, trow.data(), sizeof(int) * trow.size()));
-    RB_TRY(up(F->d_tperm, tperm.data(), sizeof(unsigned short) * tperm.size()));
-    RB_TRY(up(F->d_tinvd, tinvd.data(), sizeof(double) * tinvd.size()));
+    // slot -> permuted row (or -1), turned into the tile tables below
+    RB_TRY(up(F->d_trow, srow.data(), sizeof(int) * srow.size()));
     RB_TRY(up(F->d_gsrc, gsrc.data(), sizeof(int) * gsrc.size()));
-    RB_TRY(up(F->d_gval, gval.data(), sizeof(double) * gval.size()));
+    RB_TRY(up(F->d_gval, gval.data(), sizeof(int2) * gval.size()));
     RB_TRY(up(F->d_gmeta, gmeta.data(), sizeof(int) * gmeta.size()));
     RB_TRY(up(F->d_pub, pub.data(), sizeof(int) * pub.size()));
+    // the index pairs uploaded above become values, in place
+    {
+        const long nls = (long)NS * 64, nent = (long)gval.size();
+        if (K > 0)
+            k_fill_rec_values<<<(int)((nls + 255) / 256), 256, 0, s>>>(
+                F->d_rec, nls, K, kSrc, NRING, kVal, d_cval, p->d_invd);
+        k_fill_gval<<<(int)((nent + 255) / 256), 256, 0, s>>>(F->d_gval, nent,
+                                                             d_cval, p->d_invd);
+        k_tiles<<<(int)((nls + kTile - 1) / kTile), kTileThreads, 0, s>>>(
+            F->d_trow, p->d_levelItem, p->d_invd, nls, F->d_tperm, F->d_tinvd);
+        RB_HIP(hipGetLastError());
+    }
     RB_HIP(hipStreamSynchronize(s));
     auto t1 = std::chrono::steady_clock::now();
     F->build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
