@@ -40,9 +40,12 @@
 #include <array>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <thread>
 #include <vector>
@@ -997,10 +1000,13 @@ using HostVec = std::vector<T, NoInit<T>>;
 // (the plan's levelItem) is fetched only once the schedule is known to be
 // worth building; the values (d_cval and the plan's invd) never leave the
 // device: the uploaded tables carry index pairs in their place, which the
-// k_fill_* kernels turn into values.
+// k_fill_* kernels turn into values.  Everything before the upload is host
+// work on host data; go() is called before the first device call and
+// returns false when the upload is not wanted any more.
 rb_status build_fused_host(rb_plan *p, const HostVec<int> &rp,
                            const HostVec<int> &cc, const HostVec<int> &lp,
-                           const double *d_cval, int Wmax, hipStream_t s) {
+                           const double *d_cval, int Wmax, hipStream_t s,
+                           const std::function<bool()> &go) {
     auto t0 = std::chrono::steady_clock::now();
     // phase times for RB_FUSED_TRACE=1
     std::vector<std::pair<const char *, double>> marks;
@@ -1393,6 +1399,7 @@ rb_status build_fused_host(rb_plan *p, const HostVec<int> &rp,
     }
     if (pub.empty()) pub.push_back(0);
     mark("join+pub");
+    if (go && !go()) return RB_OK;
     // ---- upload
     std::unique_ptr<FusedPlan> owner(new FusedPlan());
     FusedPlan *F = owner.get();
@@ -1468,18 +1475,34 @@ rb_status build_fused_host(rb_plan *p, const HostVec<int> &rp,
 
 // Device part of the fused build: pull the permuted CSR's structure and
 // the level boundaries the host partitioner needs first.
-rb_status build_fused(rb_plan *p, const int *d_prptr, const int *d_ccol,
-                      const double *d_cval, int Wmax, hipStream_t s) {
+// A fused build in flight: the host passes run on `th` while the caller
+// goes on; the upload waits for `go` (1: upload, -1: abandon).
+struct FusedJob {
+    std::thread th;
+    std::mutex mu;
+    std::condition_variable cv;
+    int go = 0;
+    rb_status st = RB_OK;
+    std::string msg;
+    HostVec<int> rp, cc, lp;
+};
+
+rb_status build_fused_start(rb_plan *p, const int *d_prptr, const int *d_ccol,
+                            const double *d_cval, int Wmax, hipStream_t s,
+                            FusedJob **out) {
+    *out = nullptr;
     const int m = p->m, nnz = p->nnz;
     auto tp = std::chrono::steady_clock::now();
-    HostVec<int> rp((size_t)m + 1), cc((size_t)std::max(1, nnz)),
-        lp((size_t)p->nlevel + 1);
-    RB_HIP(hipMemcpyAsync(rp.data(), d_prptr, sizeof(int) * (m + 1),
+    std::unique_ptr<FusedJob> job(new FusedJob());
+    job->rp.resize((size_t)m + 1);
+    job->cc.resize((size_t)std::max(1, nnz));
+    job->lp.resize((size_t)p->nlevel + 1);
+    RB_HIP(hipMemcpyAsync(job->rp.data(), d_prptr, sizeof(int) * (m + 1),
                           hipMemcpyDeviceToHost, s));
     if (nnz > 0)
-        RB_HIP(hipMemcpyAsync(cc.data(), d_ccol, sizeof(int) * nnz,
+        RB_HIP(hipMemcpyAsync(job->cc.data(), d_ccol, sizeof(int) * nnz,
                               hipMemcpyDeviceToHost, s));
-    RB_HIP(hipMemcpyAsync(lp.data(), p->d_levelPtr,
+    RB_HIP(hipMemcpyAsync(job->lp.data(), p->d_levelPtr,
                           sizeof(int) * (p->nlevel + 1), hipMemcpyDeviceToHost,
                           s));
     RB_HIP(hipStreamSynchronize(s));
@@ -1488,7 +1511,41 @@ rb_status build_fused(rb_plan *p, const int *d_prptr, const int *d_ccol,
                 std::chrono::duration<double, std::milli>(
                     std::chrono::steady_clock::now() - tp)
                     .count());
-    return build_fused_host(p, rp, cc, lp, d_cval, Wmax, s);
+    int dev = 0;
+    RB_HIP(hipGetDevice(&dev));
+    FusedJob *J = job.get();
+    J->th = std::thread([p, J, d_cval, Wmax, s, dev]() {
+        // the upload runs here, on the caller's device and stream, while
+        // the caller waits in build_fused_finish
+        if (hipSetDevice(dev) != hipSuccess) {
+            J->st = status_of_hip(hipGetLastError(), "hipSetDevice");
+            J->msg = rb_last_error();
+            return;
+        }
+        J->st = build_fused_host(p, J->rp, J->cc, J->lp, d_cval, Wmax, s, [J]() {
+            std::unique_lock<std::mutex> lk(J->mu);
+            J->cv.wait(lk, [J] { return J->go != 0; });
+            return J->go > 0;
+        });
+        if (J->st != RB_OK) J->msg = rb_last_error();
+    });
+    *out = job.release();
+    return RB_OK;
+}
+
+rb_status build_fused_finish(FusedJob *job, bool upload) {
+    {
+        std::lock_guard<std::mutex> lk(job->mu);
+        job->go = upload ? 1 : -1;
+    }
+    job->cv.notify_all();
+    job->th.join();
+    const rb_status st = job->st;
+    const std::string msg = job->msg;
+    delete job;
+    if (!upload) return RB_OK;  // the caller's own error stands
+    if (st != RB_OK) set_error(st, msg);
+    return st;
 }
 
 int fused_max_waves() {

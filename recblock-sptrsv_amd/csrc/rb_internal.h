@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstddef>
 #include <cstdint>
 #include <string>
@@ -38,11 +39,27 @@ constexpr int kLongRow = 2048;  // LONGROW_THRESHOLD, R/utils_spmv_cuda.h:9
 struct Arena {
     std::vector<void *> ptrs;
     size_t bytes = 0;
+    // chunk > 0: buffers are carved (256-byte aligned) out of chunks of at
+    // least `chunk` bytes, each next chunk twice the last, so a build's
+    // temporaries cost a few hipMalloc / hipFree calls instead of one each
+    // (every hipFree waits for the device)
+    size_t chunk = 0;
+    char *cur = nullptr, *end = nullptr;
+    Arena() = default;
+    explicit Arena(size_t first_chunk) : chunk(first_chunk) {}
     template <class T>
     rb_status alloc(T **p, size_t count) {
         *p = nullptr;
         if (count == 0) count = 1;
-        hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+        const size_t need = (count * sizeof(T) + 255) & ~(size_t)255;
+        if (chunk > 0 && cur && (size_t)(end - cur) >= need) {
+            *p = (T *)cur;
+            cur += need;
+            bytes += count * sizeof(T);
+            return RB_OK;
+        }
+        const size_t got = chunk > 0 ? std::max(need, chunk) : count * sizeof(T);
+        hipError_t e = hipMalloc((void **)p, got);
         if (e != hipSuccess) {
             (void)hipGetLastError();
             set_error(RB_ERR_NO_MEMORY, "hipMalloc failed");
@@ -50,12 +67,18 @@ struct Arena {
         }
         ptrs.push_back(*p);
         bytes += count * sizeof(T);
+        if (chunk > 0) {
+            cur = (char *)*p + need;
+            end = (char *)*p + got;
+            chunk *= 2;
+        }
         return RB_OK;
     }
     void release() {
         for (void *p : ptrs) (void)hipFree(p);
         ptrs.clear();
         bytes = 0;
+        cur = end = nullptr;
     }
 };
 
@@ -230,9 +253,21 @@ rb_status solve_f32ir(rb_plan *p, const double *d_b, int ldb, double *d_x,
 // blocks keep the rule) and set *cut
 rb_status tune_blocks(rb_plan *p, int mode, hipStream_t s,
                       double budget_ms = 0.0, bool *cut = nullptr);
-// fused schedule (rb_fused.hip)
-rb_status build_fused(rb_plan *p, const int *d_prptr, const int *d_ccol,
-                      const double *d_cval, int Wmax, hipStream_t s);
+// fused schedule (rb_fused.hip): build_fused_start copies the permuted
+// CSR structure (d_prptr, d_ccol) and the level pointers to the host and
+// starts the host passes on a thread; build_fused_finish(job, true) then
+// uploads the schedule on the same stream (d_cval, the plan's levelItem
+// and invd must be ready by then) and joins the thread, or with false
+// abandons it.  Between the two calls the thread makes no device call and
+// reads only the host copies and plan fields that stay fixed (m, nnz,
+// nlevel, subst, want_schedule, tune_rhs); its allocations from the plan's
+// arena and its work on the stream happen inside build_fused_finish, while
+// the caller waits.
+struct FusedJob;
+rb_status build_fused_start(rb_plan *p, const int *d_prptr, const int *d_ccol,
+                            const double *d_cval, int Wmax, hipStream_t s,
+                            FusedJob **job);
+rb_status build_fused_finish(FusedJob *job, bool upload);
 rb_status solve_fused(rb_plan *p, const double *d_b, int ldb, double *d_x,
                       int ldx, int nrhs, hipStream_t s);
 void fused_destroy(rb_plan *p);

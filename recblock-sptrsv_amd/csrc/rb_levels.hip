@@ -591,7 +591,7 @@ rb_status bfs_blocks(const int *d_colptr, const int *d_rowidx,
                      const int *d_bms, const int *d_lvbase, int *d_item,
                      int *d_lvptr, int *d_nlv, hipStream_t s) {
     if (m == 0 || nblocks == 0) return RB_OK;
-    Arena tmp;
+    Arena tmp(12 * (size_t)m + (1 << 20));  // rem, par, cnt and counters
     int *rem, *par, *cnt, *stalled;
     RB_TRY(tmp.alloc(&rem, m));
     RB_TRY(tmp.alloc(&par, m));
@@ -642,7 +642,7 @@ rb_status bfs_levels(const int *d_colptr, const int *d_rowidx,
     if (grid > need) grid = need;
     if (grid < 1) grid = 1;
 
-    Arena tmp;
+    Arena tmp(12 * (size_t)m + (1 << 20));  // rem, par, cnt and counters
     int *rem, *par, *cnt, *blocksum;
     BfsCtl *ctl;
     RB_TRY(tmp.alloc(&rem, m));

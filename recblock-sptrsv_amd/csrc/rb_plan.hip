@@ -431,7 +431,19 @@ rb_status build_plan(rb_plan *p, const int *d_colptr, const int *d_rowidx,
                      const double *d_val, hipStream_t s) {
     auto t0 = std::chrono::steady_clock::now();
     const int m = p->m, nnz = p->nnz;
-    Arena tmp;  // released at the end
+    // RB_PLAN_TRACE=1: host wall-clock ms at the end of each step (kernels
+    // are asynchronous: their time shows at the step that next waits)
+    const bool trace = getenv("RB_PLAN_TRACE") != nullptr;
+    std::vector<std::pair<const char *, double>> marks;
+    auto mark = [&](const char *what) {
+        if (trace)
+            marks.emplace_back(what, std::chrono::duration<double, std::milli>(
+                                         std::chrono::steady_clock::now() - t0)
+                                         .count());
+    };
+    // released at the end; one chunk holds the temporaries below (about 17
+    // ints per row and 8 ints + 1 double per entry)
+    Arena tmp(68 * (size_t)m + 32 * (size_t)nnz + (1 << 20));
     struct Guard {
         Arena &a;
         ~Guard() { a.release(); }
@@ -481,6 +493,7 @@ rb_status build_plan(rb_plan *p, const int *d_colptr, const int *d_rowidx,
     }
 
     Arena &A = p->arena;
+    mark("validation");
     // ---- 1. row counts and global level analysis
     int *rowcnt, *bfs_item;
     RB_TRY(tmp.alloc(&rowcnt, m));
@@ -493,6 +506,7 @@ rb_status build_plan(rb_plan *p, const int *d_colptr, const int *d_rowidx,
                       &p->nlevel, s));
     auto t_lv = std::chrono::steady_clock::now();
 
+    mark("levels");
     // ---- 2. level-order permutation of the CSC
     int *newpos, *pcolcnt, *ecol;
     RB_TRY(A.alloc(&p->d_levelItem, m));
@@ -518,6 +532,7 @@ rb_status build_plan(rb_plan *p, const int *d_colptr, const int *d_rowidx,
         RB_HIP(hipGetLastError());
     }
 
+    mark("permutation");
     // ---- 3. permuted CSR with ascending columns (stable sort by row)
     int *iota, *sorted_rows, *order, *prowcnt, *prptr, *ccol;
     double *cval;
@@ -542,6 +557,21 @@ rb_status build_plan(rb_plan *p, const int *d_colptr, const int *d_rowidx,
     RB_HIP(hipGetLastError());
     RB_TRY(exclusive_scan_i32(prowcnt, prptr, m + 1, s));
 
+    // ---- 9, begun here: the fused schedule's host passes need only the
+    // permuted CSR structure and the levels, so they run on a thread while
+    // steps 4-8 run; the upload follows step 8
+    FusedJob *fjob = nullptr;
+    struct JobGuard {  // an early return abandons the job (joins the thread)
+        FusedJob *&j;
+        ~JobGuard() {
+            if (j) (void)build_fused_finish(j, false);
+        }
+    } jguard{fjob};
+    auto t_fs0 = std::chrono::steady_clock::now();
+    if (p->want_schedule != RB_SCHED_BLOCKED && m > 0)
+        RB_TRY(build_fused_start(p, prptr, ccol, cval, p->fused_waves, s, &fjob));
+    auto t_fs1 = std::chrono::steady_clock::now();
+    mark("csr");
     // ---- 4. partition and slot space
     p->ntri = 1 << p->lv;
     p->nblocks = 2 * p->ntri - 1;
@@ -588,6 +618,7 @@ rb_status build_plan(rb_plan *p, const int *d_colptr, const int *d_rowidx,
                                            p->d_rowptr, p->d_col, p->d_val);
     RB_HIP(hipGetLastError());
 
+    mark("slots");
     // ---- 5. block statistics (blk_nnz, cal_longrow)
     int *d_stats;
     RB_TRY(tmp.alloc(&d_stats, nb * 6));
@@ -606,6 +637,7 @@ rb_status build_plan(rb_plan *p, const int *d_colptr, const int *d_rowidx,
         }
     }
 
+    mark("block stats");
     // ---- 6. DCSR row lists and long-row lists of square blocks
     {
         int *kind, *local, *fd, *fl, *pd, *pl;
@@ -645,6 +677,7 @@ rb_status build_plan(rb_plan *p, const int *d_colptr, const int *d_rowidx,
         }
     }
 
+    mark("dcsr");
     // ---- 7. local level analysis of the triangular blocks: each block's
     // own BFS over its in-block entries (the block-diagonal matrix D), all
     // blocks at once, one workgroup each
@@ -726,6 +759,7 @@ rb_status build_plan(rb_plan *p, const int *d_colptr, const int *d_rowidx,
         RB_HIP(hipStreamSynchronize(s));
     }
 
+    mark("local levels");
     // ---- 8. selector (R/recblocking_solver_cuda.h:176-367, :441-546)
     for (int i = 0; i < nb; i++) {
         Block &b = p->blocks[i];
@@ -771,12 +805,17 @@ rb_status build_plan(rb_plan *p, const int *d_colptr, const int *d_rowidx,
         }
     }
 
+    mark("selector");
     // ---- 9. fused schedule storage (chain-tiled partition of the rows)
     auto t_f0 = std::chrono::steady_clock::now();
-    if (p->want_schedule != RB_SCHED_BLOCKED && m > 0)
-        RB_TRY(build_fused(p, prptr, ccol, cval, p->fused_waves, s));
+    if (fjob) {
+        FusedJob *j = fjob;
+        fjob = nullptr;
+        RB_TRY(build_fused_finish(j, true));
+    }
     auto t_f1 = std::chrono::steady_clock::now();
 
+    mark("fused");
     // ---- 10. FP32 + FP64-refinement storage: FP32 copies of the slot
     // values, and the level-ordered CSR with original ids (row i is
     // original row levelItem[i]) for FP64 residuals in the original order
@@ -802,6 +841,7 @@ rb_status build_plan(rb_plan *p, const int *d_colptr, const int *d_rowidx,
         RB_TRY(A.alloc(&p->d_xperm32, m));
     }
 
+    mark("fp32 copies");
     // ---- 11. solve scratch
     RB_TRY(A.alloc(&p->d_bperm, m));
     RB_TRY(A.alloc(&p->d_xperm, m));
@@ -818,8 +858,16 @@ rb_status build_plan(rb_plan *p, const int *d_colptr, const int *d_rowidx,
         return std::chrono::duration<double, std::milli>(b - a).count();
     };
     p->phase_ms[0] = ms(t0, t_lv);
-    p->phase_ms[2] = ms(t_f0, t_f1);
+    // the fused build's own share: structure copies, then the wait for its
+    // host passes and the upload (the passes overlap steps 4-8)
+    p->phase_ms[2] = ms(t_fs0, t_fs1) + ms(t_f0, t_f1);
     p->phase_ms[1] = p->preprocess_ms - p->phase_ms[0] - p->phase_ms[2];
+    mark("scratch");
+    if (trace) {
+        fprintf(stderr, "plan build (cumulative ms):");
+        for (auto &mk : marks) fprintf(stderr, " %s %.1f", mk.first, mk.second);
+        fprintf(stderr, " (%zu temporary buffers)\n", tmp.ptrs.size());
+    }
     return RB_OK;
 }
 
