@@ -139,6 +139,14 @@ __device__ int narrow_scan(int *cl, int len, int *lds) {
     return total;
 }
 
+// The frontier of the next level, kept in LDS by a cached level when that
+// level is cached too: its nodes and their column ranges [a, b).
+struct Front {
+    int u[kIlp * kBfsThreads];
+    int a[kIlp * kBfsThreads];
+    int b[kIlp * kBfsThreads];
+};
+
 // One narrow level [s, e) of at most kIlp * kBfsThreads nodes.  Each thread
 // keeps its nodes, their entry ranges and the first kE entries of their
 // columns in registers from phase A to C, and phase B remembers which of
@@ -147,19 +155,31 @@ __device__ int narrow_scan(int *cl, int len, int *lds) {
 // Entries past the first kE of a column are re-read (the generic rule).
 __device__ int level_cached(const int *__restrict__ colptr,
                             const int *__restrict__ rowidx, int *rem, int *par,
-                            int *levelItem, int s, int e, int *cl, int *lds) {
+                            int *levelItem, int s, int e, int *cl, int *lds,
+                            Front &fr, bool &held) {
     const int tid = threadIdx.x, len = e - s;
+    const bool in = held;  // this level's nodes and column ranges are in fr
     int u[kIlp], q0[kIlp], q1[kIlp], vc[kIlp][kE];
     unsigned kid[kIlp];
+    if (in) {
 #pragma unroll
-    for (int k = 0; k < kIlp; k++) {
-        const int i = tid + k * kBfsThreads;
-        u[k] = i < len ? ld_wg(&levelItem[s + i]) : -1;
-    }
+        for (int k = 0; k < kIlp; k++) {
+            const int i = tid + k * kBfsThreads;
+            u[k] = i < len ? fr.u[i] : -1;
+            q0[k] = i < len ? fr.a[i] : 0;
+            q1[k] = i < len ? fr.b[i] : 0;
+        }
+    } else {
 #pragma unroll
-    for (int k = 0; k < kIlp; k++) {
-        q0[k] = u[k] >= 0 ? colptr[u[k]] : 0;
-        q1[k] = u[k] >= 0 ? colptr[u[k] + 1] : 0;
+        for (int k = 0; k < kIlp; k++) {
+            const int i = tid + k * kBfsThreads;
+            u[k] = i < len ? ld_wg(&levelItem[s + i]) : -1;
+        }
+#pragma unroll
+        for (int k = 0; k < kIlp; k++) {
+            q0[k] = u[k] >= 0 ? colptr[u[k]] : 0;
+            q1[k] = u[k] >= 0 ? colptr[u[k] + 1] : 0;
+        }
     }
 #pragma unroll
     for (int k = 0; k < kIlp; k++)
@@ -193,11 +213,13 @@ __device__ int level_cached(const int *__restrict__ colptr,
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     // B: children (rem 0, parent p) per node.  The cached entries' rem and
     // par are read together (a non-candidate reads row 0, in range, and is
-    // ignored).  A narrow level costs about five dependent trips to memory
-    // (frontier, column pointers, entries, atomics, rem / par) at ~2 us
-    // each on the 1500^2 Laplacian; reading rem / par entry by entry behind
-    // branches cost another ~0.5 us.
-    int rv[kIlp][kE], pv[kIlp][kE];
+    // ignored), and with them the entries' column ranges, which a child
+    // takes into the next level's frontier in LDS.  A narrow level cost
+    // about five dependent trips to memory (frontier, column pointers,
+    // entries, atomics, rem / par) at ~2 us each on the 1500^2 Laplacian;
+    // with the frontier in LDS three are left.  Reading rem / par entry by
+    // entry behind branches cost another ~0.5 us.
+    int rv[kIlp][kE], pv[kIlp][kE], ca[kIlp][kE], cb[kIlp][kE];
 #pragma unroll
     for (int k = 0; k < kIlp; k++)
 #pragma unroll
@@ -206,6 +228,8 @@ __device__ int level_cached(const int *__restrict__ colptr,
             const int va = v >= 0 && v != u[k] ? v : 0;
             rv[k][j] = ld_wg(&rem[va]);
             pv[k][j] = ld_wg(&par[va]);
+            ca[k][j] = colptr[va];
+            cb[k][j] = colptr[va + 1];
         }
 #pragma unroll
     for (int k = 0; k < kIlp; k++) {
@@ -231,7 +255,10 @@ __device__ int level_cached(const int *__restrict__ colptr,
     }
     __syncthreads();
     const int total = narrow_scan(cl, len, lds);
-    // C: children in column order at the node's offset
+    // C: children in column order at the node's offset; when the next
+    // level is cached too, its frontier also goes to fr (every read of fr
+    // was in A, before this level's barriers)
+    const bool out = total <= kIlp * kBfsThreads;
 #pragma unroll
     for (int k = 0; k < kIlp; k++) {
         const int i = tid + k * kBfsThreads;
@@ -240,16 +267,29 @@ __device__ int level_cached(const int *__restrict__ colptr,
         int o = e + cl[i];
 #pragma unroll
         for (int j = 0; j < kE; j++)
-            if ((kid[k] >> j) & 1u)
+            if ((kid[k] >> j) & 1u) {
+                if (out) {
+                    fr.u[o - e] = vc[k][j];
+                    fr.a[o - e] = ca[k][j];
+                    fr.b[o - e] = cb[k][j];
+                }
                 __hip_atomic_store(&levelItem[o++], vc[k][j], __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
         for (int q = q0[k] + kE; q < q1[k]; q++) {
             const int v = rowidx[q];
-            if (v != u[k] && ld_wg(&rem[v]) == 0 && ld_wg(&par[v]) == p)
+            if (v != u[k] && ld_wg(&rem[v]) == 0 && ld_wg(&par[v]) == p) {
+                if (out) {
+                    fr.u[o - e] = v;
+                    fr.a[o - e] = colptr[v];
+                    fr.b[o - e] = colptr[v + 1];
+                }
                 __hip_atomic_store(&levelItem[o++], v, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
         }
     }
+    held = out;
     return total;
 }
 
@@ -308,15 +348,21 @@ __device__ void bfs_narrow(const int *__restrict__ colptr,
                            int *par, int *levelItem, int *levelPtr,
                            BfsCtl *ctl, int *lds) {
     __shared__ int cl[kNarrow];
+    __shared__ Front fr;
     const int tid = threadIdx.x;
     int s = ld_relaxed(&ctl->s), e = ld_relaxed(&ctl->e);
     int nl = ld_relaxed(&ctl->nlevel);
     int stalled = 0;
+    bool held = false;  // the level's frontier is in fr
     while (e < m && e - s <= kNarrow) {
-        const int total =
-            e - s <= kIlp * kBfsThreads
-                ? level_cached(colptr, rowidx, rem, par, levelItem, s, e, cl, lds)
-                : level_generic(colptr, rowidx, rem, par, levelItem, s, e, cl, lds);
+        int total;
+        if (e - s <= kIlp * kBfsThreads) {
+            total = level_cached(colptr, rowidx, rem, par, levelItem, s, e, cl, lds,
+                                 fr, held);
+        } else {
+            total = level_generic(colptr, rowidx, rem, par, levelItem, s, e, cl, lds);
+            held = false;
+        }
         if (tid == 0) levelPtr[nl + 1] = e + total;
         nl++;
         s = e;
@@ -510,6 +556,7 @@ k_bfs_blocks(const int *__restrict__ colptr, const int *__restrict__ rowidx,
              int *stalled) {
     __shared__ int lds[16];
     __shared__ int cl[kNarrow];
+    __shared__ Front fr;
     const int t = blockIdx.x, tid = threadIdx.x;
     const int up = ups[t], bm = bms[t];
     int *lp = lvptr + lvbase[t];
@@ -549,14 +596,18 @@ k_bfs_blocks(const int *__restrict__ colptr, const int *__restrict__ rowidx,
     }
     int s = 0, nl = 1;
     bool bad = e == 0;
+    bool held = false;  // the level's frontier is in fr
     while (e < bm && !bad) {
         const int len = e - s;
-        const int total =
-            len <= kIlp * kBfsThreads
-                ? level_cached(colptr, rowidx, rem, par, it, s, e, cl, lds)
-            : len <= kNarrow
-                ? level_generic(colptr, rowidx, rem, par, it, s, e, cl, lds)
-                : level_wide(colptr, rowidx, rem, par, it, s, e, cnt + up, lds);
+        int total;
+        if (len <= kIlp * kBfsThreads) {
+            total = level_cached(colptr, rowidx, rem, par, it, s, e, cl, lds, fr, held);
+        } else {
+            total = len <= kNarrow
+                        ? level_generic(colptr, rowidx, rem, par, it, s, e, cl, lds)
+                        : level_wide(colptr, rowidx, rem, par, it, s, e, cnt + up, lds);
+            held = false;
+        }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __syncthreads();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
