@@ -96,6 +96,39 @@ def _median_time(fn, budget_s, min_reps=20, warm=3):
     return float(np.median(ts)), len(ts)
 
 
+def f32ir_leg(args, dcp, dri, dv, f, db, K, fp64_ms):
+    """FP32 solves refined in FP64 to 1e-12 (rb_solve_f32ir) on the same
+    factor and right-hand sides: a plan built with mixed=True under the
+    same schedule rule, timed over args.steps calls with their report (one
+    host synchronisation per call, after the second correction).  Beside
+    the plain FP64 line, not instead of it."""
+    import numpy as np
+    import torch
+    from rbsptrsv import Plan
+    q = Plan(dcp, dri, dv, f.substitution, -1, schedule=args.schedule,
+             fused_waves=args.fused_waves, mixed=True)
+    x, it, rel = q.solve_f32ir(db, tol=1e-12)
+    for _ in range(args.warmup):
+        q.solve_f32ir(db, x=x, tol=1e-12)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        x, it, rel = q.solve_f32ir(db, x=x, tol=1e-12)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    q.check()
+    sched = {1: "blocked", 2: "fused"}.get(q.info().schedule, q.info().schedule)
+    q.close()
+    X = x.t().cpu().numpy() if K > 1 else x.cpu().numpy()[None, :]
+    B = db.t().cpu().numpy() if K > 1 else db.cpu().numpy()[None, :]
+    res = max(float(np.linalg.norm(B[j] - f.matvec(X[j])) /
+                    np.linalg.norm(B[j])) for j in range(K))
+    return {"ms_per_call": round(ms, 4), "solves_per_s": round(K / (ms * 1e-3), 1),
+            "vs_fp64_solve": round(fp64_ms / ms, 3), "corrections": it,
+            "relres_max": rel, "residual": res, "tol": 1e-12,
+            "schedule": sched, "dtype": "f32 solves, f64 residual and update"}
+
+
 def cpu_baseline(f, lv, budget_s):
     """The reference's algorithm on the host cores: the oracle's level
     reorder, partition and block order, triangular blocks level by level
@@ -369,6 +402,9 @@ def main():
                     help="right-hand sides per solve call (column-major)")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--f32ir", action="store_true",
+                    help="single-factor workloads: also time FP32 solves "
+                         "refined in FP64 to 1e-12 on the same factor")
     ap.add_argument("--no-plain", action="store_true",
                     help="batch workload: skip the one-plan-per-factor path")
     ap.add_argument("--rate", type=float, default=16000.0,
@@ -536,6 +572,8 @@ def main():
                          "phases_ms": {k: round(info_w.phase_ms[i], 1) for i, k in
                                        enumerate(("levels", "blocks",
                                                   "fused_schedule", "tuning"))}}
+    if args.f32ir:
+        out["f32ir"] = f32ir_leg(args, dcp, dri, dv, f, db, K, ms)
     plan.close()    # explicitly, before the interpreter (and any profiler
                     # attached to the HIP runtime) shuts down
     if not args.no_cpu:

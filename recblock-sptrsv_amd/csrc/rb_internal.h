@@ -221,7 +221,7 @@ struct rb_plan {
     float *d_val32 = nullptr, *d_invd32 = nullptr;   // slot values in FP32
     float *d_bperm32 = nullptr, *d_xperm32 = nullptr;
     int *d_rptr = nullptr, *d_rcol = nullptr;        // FP64 matrix for the
-    double *d_rval = nullptr;                        // residual (see below)
+    double *d_rval = nullptr;                        // residual: CSR, original order
     // refinement scratch for kir right-hand sides (allocated on first use)
     int kir = 0;
     double *d_r = nullptr;                           // residual, FP64

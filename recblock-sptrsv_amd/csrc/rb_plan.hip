@@ -140,10 +140,25 @@ __global__ void k_pcolcopy(const int *__restrict__ colptr,
 }
 
 // column ids of the level-ordered CSR mapped back to original ids
-__global__ void k_orig_cols(const int *__restrict__ ccol,
-                            const int *__restrict__ lf, int nnz, int *rcol) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < nnz) rcol[k] = lf[ccol[k]];
+// The CSR of the original matrix, from the permuted one (row r of which is
+// original row lf[r]): row lengths into original order, then each row's
+// entries with original column ids, in the permuted row's entry order.
+__global__ void k_orig_len(const int *__restrict__ prptr, const int *__restrict__ lf,
+                           int m, int *olen) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < m) olen[lf[r]] = prptr[r + 1] - prptr[r];
+}
+__global__ void k_orig_rows(const int *__restrict__ prptr, const int *__restrict__ ccol,
+                            const double *__restrict__ cval, const int *__restrict__ lf,
+                            int m, const int *__restrict__ optr, int *ocol,
+                            double *oval) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= m) return;
+    int d = optr[lf[r]];
+    for (int k = prptr[r]; k < prptr[r + 1]; k++, d++) {
+        ocol[d] = lf[ccol[k]];
+        oval[d] = cval[k];
+    }
 }
 
 __global__ void k_f64_to_f32(const double *__restrict__ a, long n, float *o) {
@@ -817,8 +832,10 @@ rb_status build_plan(rb_plan *p, const int *d_colptr, const int *d_rowidx,
 
     mark("fused");
     // ---- 10. FP32 + FP64-refinement storage: FP32 copies of the slot
-    // values, and the level-ordered CSR with original ids (row i is
-    // original row levelItem[i]) for FP64 residuals in the original order
+    // values, and the original matrix as CSR for the FP64 residuals (rows,
+    // b, r and the x gathers all in the caller's order: the level-ordered
+    // rows put neighbouring rows of a grid ~n apart, every access on a
+    // cache line of its own)
     if (p->mixed && m > 0) {
         RB_TRY(A.alloc(&p->d_val32, offnnz));
         RB_TRY(A.alloc(&p->d_invd32, m));
@@ -828,14 +845,14 @@ rb_status build_plan(rb_plan *p, const int *d_colptr, const int *d_rowidx,
         RB_TRY(A.alloc(&p->d_rptr, m + 1));
         RB_TRY(A.alloc(&p->d_rcol, nnz));
         RB_TRY(A.alloc(&p->d_rval, nnz));
-        RB_HIP(hipMemcpyAsync(p->d_rptr, prptr, sizeof(int) * (size_t)(m + 1),
-                              hipMemcpyDeviceToDevice, s));
-        if (nnz > 0) {
-            k_orig_cols<<<nblk(nnz), kT, 0, s>>>(ccol, p->d_levelItem, nnz,
-                                                 p->d_rcol);
-            RB_HIP(hipMemcpyAsync(p->d_rval, cval, sizeof(double) * (size_t)nnz,
-                                  hipMemcpyDeviceToDevice, s));
-        }
+        int *olen;
+        RB_TRY(tmp.alloc(&olen, m + 1));
+        RB_HIP(hipMemsetAsync(olen + m, 0, sizeof(int), s));
+        k_orig_len<<<nblk(m), kT, 0, s>>>(prptr, p->d_levelItem, m, olen);
+        RB_HIP(hipGetLastError());
+        RB_TRY(exclusive_scan_i32(olen, p->d_rptr, m + 1, s));
+        k_orig_rows<<<nblk(m), kT, 0, s>>>(prptr, ccol, cval, p->d_levelItem, m,
+                                           p->d_rptr, p->d_rcol, p->d_rval);
         RB_HIP(hipGetLastError());
         RB_TRY(A.alloc(&p->d_bperm32, m));
         RB_TRY(A.alloc(&p->d_xperm32, m));

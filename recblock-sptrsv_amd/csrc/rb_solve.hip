@@ -585,9 +585,8 @@ __global__ void k_ir_update(const float *__restrict__ d, int m, double *x,
     if (!done[c] && i < m) x[c * ldx + i] += (double)d[(long)c * m + i];
 }
 
-// r = b - A x in the original ordering, FP64: row i of the level-ordered
-// CSR is original row lf[i], its columns hold original ids; max |r| per
-// column.  G lanes per row (a power of two at or above the mean row
+// r = b - A x in the original ordering, FP64, over the original matrix as
+// CSR (rows, b, r and x all in the caller's order); max |r| per column.  G lanes per row (a power of two at or above the mean row
 // length, as for the square blocks' SpMV) read the row's entries coalesced
 // and sum them in a fixed order (a thread per row was slow on the
 // power-law factor, whose rows reach 512 entries), for KC columns at once
@@ -599,8 +598,7 @@ __global__ void k_ir_update(const float *__restrict__ d, int m, double *x,
 template <int G, int KC>
 __global__ void __launch_bounds__(kT)
 k_ir_residual(const int *__restrict__ rptr, const int *__restrict__ rcol,
-              const double *__restrict__ rval, const int *__restrict__ lf,
-              int m, int ncol, const double *__restrict__ b, long ldb,
+              const double *__restrict__ rval, int m, int ncol, const double *__restrict__ b, long ldb,
               const double *__restrict__ x, long ldx, double *r,
               unsigned long long *rmax, const int *__restrict__ done) {
     const int c0 = blockIdx.y * KC;
@@ -638,13 +636,12 @@ k_ir_residual(const int *__restrict__ rptr, const int *__restrict__ rcol,
             for (int o = G / 2; o > 0; o >>= 1) acc[q] += __shfl_xor(acc[q], o, G);
         }
         if (g == 0) {
-            const int o = lf[i];
 #pragma unroll
             for (int q = 0; q < KC; q++)
                 if (live[q]) {
                     const int c = c0 + q;
-                    const double v = b[c * ldb + o] - acc[q];
-                    r[(long)c * m + o] = v;
+                    const double v = b[c * ldb + i] - acc[q];
+                    r[(long)c * m + i] = v;
                     a[q] = fmax(a[q], fabs(v));
                 }
         }
@@ -668,7 +665,7 @@ static void launch_residual(rb_plan *p, int nc, const double *b, long ldb,
     const int m = p->m;
     const int grid = (int)std::min<long>(1024, nblk((long)m * G));
     k_ir_residual<G, KC><<<dim3(grid, (nc + KC - 1) / KC), kT, 0, s>>>(
-        p->d_rptr, p->d_rcol, p->d_rval, p->d_levelItem, m, nc, b, ldb, x,
+        p->d_rptr, p->d_rcol, p->d_rval, m, nc, b, ldb, x,
         ldx, p->d_r, p->d_norms + nc, p->d_irdone);
 }
 
