@@ -554,8 +554,13 @@ void launch_sq(int g, int nrows, int ncol, hipStream_t s, const int *rowptr,
 // columns that are done skip every later pass.
 
 // r = b, x = 0, max |b|
-__global__ void k_ir_init(const double *__restrict__ b, long ldb, double *x,
-                          long ldx, double *r, int m, unsigned long long *bmax) {
+// x = 0, r = b and max |b| of column blockIdx.y: the maxima are folded
+// per workgroup, one atomic each (an atomic per wave on the column's one
+// word serialised: 55 us on 2.25M rows)
+__global__ void __launch_bounds__(kT)
+    k_ir_init(const double *__restrict__ b, long ldb, double *x, long ldx,
+              double *r, int m, unsigned long long *bmax) {
+    __shared__ double wmax[kT / 64];
     const int c = blockIdx.y;
     double mx = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m;
@@ -566,8 +571,12 @@ __global__ void k_ir_init(const double *__restrict__ b, long ldb, double *x,
         mx = fmax(mx, fabs(v));
     }
     mx = wave_max(mx);
-    if ((threadIdx.x & 63) == 0)
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kT / 64; w++) mx = fmax(mx, wmax[w]);
         atomicMax(&bmax[c], (unsigned long long)__double_as_longlong(mx));
+    }
 }
 
 __global__ void k_ir_to32(const double *__restrict__ r, int m, float *r32,
