@@ -733,25 +733,27 @@ constexpr int kTile = 1024, kTileThreads = 256;
 // tile offsets sort in the low 12 bits of a key and fit an unsigned short
 static_assert(kTile <= 4096 && (kTile & (kTile - 1)) == 0, "kTile");
 
-// (left: as for k_fused, nothing to do when *left == 0)
-template <class T>
+// (left: as for k_fused, nothing to do when *left == 0; b of type B is
+// rounded to T first: the FP32 solves of the refinement read the FP64
+// residual)
+template <class T, class B = T>
 __global__ void __launch_bounds__(kTileThreads)
     k_fused_prologue(const int *__restrict__ trow,
                      const unsigned short *__restrict__ tperm,
                      const T *__restrict__ tinvd, long nslots,
-                     const T *__restrict__ b, long ldb, T *brec,
+                     const B *__restrict__ b, long ldb, T *brec,
                      const int *__restrict__ left) {
     if (left && *left == 0) return;
     __shared__ T buf[kTile];
     const long base = (long)blockIdx.x * kTile;
     const int cnt = (int)min((long)kTile, nslots - base);
-    const T *br = b + blockIdx.y * ldb;
+    const B *br = b + blockIdx.y * ldb;
 #pragma unroll
     for (int k = 0; k < kTile / kTileThreads; k++) {
         const int i = k * kTileThreads + threadIdx.x;
         if (i < cnt) {
             const int row = trow[base + i];
-            buf[tperm[base + i]] = row >= 0 ? br[row] * tinvd[base + i] : T(0);
+            buf[tperm[base + i]] = row >= 0 ? T(br[row]) * tinvd[base + i] : T(0);
         }
     }
     __syncthreads();
@@ -789,6 +791,43 @@ __global__ void __launch_bounds__(kTileThreads)
         if (i < cnt) {
             const int row = trow[base + i];
             if (row >= 0) xr[row] = buf[tperm[base + i]];
+        }
+    }
+}
+
+// The refinement's epilogue on the FP32 schedule: x (FP64, caller's order)
+// += the FP32 correction of every column not yet converged (done[c] == 0),
+// and every slot read goes back to the sentinel, so the buffers are ready
+// for the next correction without a pass of their own.
+__global__ void __launch_bounds__(kTileThreads)
+    k_fused_epilogue_ir(const int *__restrict__ trow,
+                        const unsigned short *__restrict__ tperm, long nslots,
+                        float *xq, double *x, long ldx, const int *status,
+                        const int *__restrict__ done,
+                        const int *__restrict__ left) {
+    if (left && *left == 0) return;
+    __shared__ float buf[kTile];
+    const long base = (long)blockIdx.x * kTile;
+    const int cnt = (int)min((long)kTile, nslots - base);
+    float *in = xq + blockIdx.y * nslots + base;
+    const bool tripped = (*status & 7) != 0;
+#pragma unroll
+    for (int k = 0; k < kTile / kTileThreads; k++) {
+        const int i = k * kTileThreads + threadIdx.x;
+        if (i < cnt) {
+            buf[i] = tripped ? __builtin_nanf("") : in[i];
+            in[i] = __uint_as_float(kPending32);
+        }
+    }
+    __syncthreads();
+    if (done[blockIdx.y]) return;  // the whole workgroup
+    double *xr = x + blockIdx.y * ldx;
+#pragma unroll
+    for (int k = 0; k < kTile / kTileThreads; k++) {
+        const int i = k * kTileThreads + threadIdx.x;
+        if (i < cnt) {
+            const int row = trow[base + i];
+            if (row >= 0) xr[row] += (double)buf[tperm[base + i]];
         }
     }
 }
@@ -921,6 +960,7 @@ struct FusedPlan {
     float *d_gval32 = nullptr, *d_tinvd32 = nullptr;
     float *d_xq32 = nullptr, *d_brec32 = nullptr;
     int nr32 = 0;
+    bool armed32 = false;  // every FP32 slot holds the sentinel
     int ncu = 0, per_cu = 0;   // CUs; k_fused workgroups resident per CU
     bool small = false;        // single-column solves are ordinary launches
     unsigned long long *d_trace = nullptr;  // RB_FUSED_TRACE=1
@@ -1661,11 +1701,15 @@ static int rhs_per_launch(FusedPlan *F) {
 // next solve, whose published slots the launch re-arms (may be null); brec:
 // nr slot-order b/diag buffers.  Buffers of one right-hand side are nslots
 // apart.
-// T: FP64, or FP32 on the FP32 tables (left: see k_fused).
+// T: FP64, or FP32 on the FP32 tables for the refinement (left: see
+// k_fused): then b is the FP64 residual and x (FP64) receives x += the
+// correction in its not converged columns (done) and cur is refilled with
+// the sentinel.
 template <class T>
-static rb_status launch_group(rb_plan *p, const T *b, int ldb, T *x, int ldx,
-                              int nr, T *cur, T *nxt, T *brec, hipStream_t s,
-                              const int *left = nullptr) {
+static rb_status launch_group(rb_plan *p, const double *b, int ldb, double *x,
+                              int ldx, int nr, T *cur, T *nxt, T *brec,
+                              hipStream_t s, const int *left = nullptr,
+                              const int *done = nullptr) {
     FusedPlan *F = p->fused;
     constexpr bool f32 = sizeof(T) == 4;
     const size_t nslots = (size_t)F->nsteps * 64;
@@ -1678,7 +1722,7 @@ static rb_status launch_group(rb_plan *p, const T *b, int ldb, T *x, int ldx,
         tinvd = F->d_tinvd;
         gval = F->d_gval;
     }
-    k_fused_prologue<T><<<tiles, kTileThreads, 0, s>>>(
+    k_fused_prologue<T, double><<<tiles, kTileThreads, 0, s>>>(
         F->d_trow, F->d_tperm, tinvd, (long)nslots, b, ldb, brec, left);
     RB_HIP(hipGetLastError());
     FusedArgsT<T> args{F->d_step0, F->d_gent0, F->d_pub0, F->d_rec,
@@ -1698,8 +1742,13 @@ static rb_status launch_group(rb_plan *p, const T *b, int ldb, T *x, int ldx,
         return status_of_hip(e, ordinary ? "hipLaunchKernel(k_fused)"
                                          : "hipLaunchCooperativeKernel(k_fused)");
     mark_end(p, s);
-    k_fused_epilogue<T><<<tiles, kTileThreads, 0, s>>>(
-        F->d_trow, F->d_tperm, (long)nslots, cur, x, ldx, p->d_status, left);
+    if constexpr (f32)
+        k_fused_epilogue_ir<<<tiles, kTileThreads, 0, s>>>(
+            F->d_trow, F->d_tperm, (long)nslots, cur, x, ldx, p->d_status, done,
+            left);
+    else
+        k_fused_epilogue<T><<<tiles, kTileThreads, 0, s>>>(
+            F->d_trow, F->d_tperm, (long)nslots, cur, x, ldx, p->d_status, left);
     RB_HIP(hipGetLastError());
     return RB_OK;
 }
@@ -1811,31 +1860,39 @@ static rb_status ensure32(rb_plan *p, int nr, hipStream_t s) {
             return RB_ERR_NO_MEMORY;
         }
         F->nr32 = nr;
+        F->armed32 = false;
+    }
+    if (!F->armed32) {  // every slot to the sentinel; the epilogues keep it so
+        const long n = (long)F->nr32 * (long)nslots;
+        k_fill_pending<float><<<(int)((n + 255) / 256), 256, 0, s>>>(F->d_xq32, n,
+                                                                    nullptr);
+        RB_HIP(hipGetLastError());
+        F->armed32 = true;
     }
     return RB_OK;
 }
 
-rb_status solve_fused32(rb_plan *p, const float *d_b, int ldb, float *d_x,
-                        int ldx, int nrhs, hipStream_t s, const int *left) {
+rb_status solve_fused32(rb_plan *p, const double *d_r, int ldr, double *d_x,
+                        int ldx, int nrhs, hipStream_t s, const int *left,
+                        const int *done) {
     FusedPlan *F = p->fused;
     if (!F) {
         set_error(RB_ERR_INTERNAL, "fused schedule not built");
         return RB_ERR_INTERNAL;
     }
     if (p->m == 0) return RB_OK;
-    const size_t nslots = (size_t)F->nsteps * 64;
     device_shape(F);
     const int cap = nrhs > 1 ? rhs_per_launch(F) : 1;
     RB_TRY(ensure32(p, std::min(nrhs, cap), s));
     for (int k0 = 0; k0 < nrhs; k0 += cap) {
         const int nr = std::min(cap, nrhs - k0);
-        // every slot back to the sentinel: nothing is re-armed
-        const long n = (long)nr * (long)nslots;
-        k_fill_pending<float><<<(int)((n + 255) / 256), 256, 0, s>>>(F->d_xq32, n, left);
-        RB_HIP(hipGetLastError());
-        RB_TRY(launch_group<float>(p, d_b + (size_t)k0 * ldb, ldb,
-                                   d_x + (size_t)k0 * ldx, ldx, nr, F->d_xq32,
-                                   nullptr, F->d_brec32, s, left));
+        const rb_status st = launch_group<float>(
+            p, d_r + (size_t)k0 * ldr, ldr, d_x + (size_t)k0 * ldx, ldx, nr,
+            F->d_xq32, nullptr, F->d_brec32, s, left, done + k0);
+        if (st != RB_OK) {
+            F->armed32 = false;  // an epilogue may not have run
+            return st;
+        }
     }
     return RB_OK;
 }

@@ -283,11 +283,14 @@ rb_status solve_blocked_ws(rb_plan *p, const double *d_b, int ldb,
 rb_status solve_fused_ws(rb_plan *p, const double *d_b, int ldb, double *d_x,
                          int ldx, int nrhs, hipStream_t s, double *ws,
                          size_t ws_bytes);
-// FP32 solve on the fused schedule (FP32 copies of its tables, made on
-// first use): the refinement's inner solver; does nothing once *left == 0
-// (left may be null)
-rb_status solve_fused32(rb_plan *p, const float *d_b, int ldb, float *d_x,
-                        int ldx, int nrhs, hipStream_t s, const int *left);
+// One correction of the FP32 refinement on the fused schedule (FP32
+// copies of its tables, made on first use): FP32 solves of the FP64
+// residual d_r (rounded to FP32 on the way in), added to the FP64 d_x in
+// every column c with done[c] == 0; does nothing once *left == 0 (left may
+// be null)
+rb_status solve_fused32(rb_plan *p, const double *d_r, int ldr, double *d_x,
+                        int ldx, int nrhs, hipStream_t s, const int *left,
+                        const int *done);
 // events around a solve's main launches when timing is on: from the start
 // of the first (the entry point sets mark_first) to the end of the last
 inline void mark_start(rb_plan *p, hipStream_t s) {

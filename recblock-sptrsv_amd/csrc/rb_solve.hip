@@ -1294,15 +1294,18 @@ rb_status solve_f32ir(rb_plan *p, const double *d_b, int ldb, double *d_x,
             b, ldb, x, ldx, p->d_r, m, p->d_norms);
         // max |b| / max |r| of column c: d_norms[c] / d_norms[nc + c]
         for (int it = 1; it <= max_iter; it++) {
-            k_ir_to32<<<g2, kT, 0, s>>>(p->d_r, m, p->d_r32, p->d_irdone);
             // once every column has converged the solve is skipped (the
             // walk's prologue sets the width its kernels read to zero; the
             // fused kernels return at once)
-            if (p->schedule == RB_SCHED_FUSED && p->fused)
-                RB_TRY(solve_fused32(p, p->d_r32, m, p->d_d32, m, nc, s, p->d_irleft));
-            else
+            if (p->schedule == RB_SCHED_FUSED && p->fused) {
+                // rounding of r, the solve and x += d in the fused passes
+                RB_TRY(solve_fused32(p, p->d_r, m, x, ldx, nc, s, p->d_irleft,
+                                     p->d_irdone));
+            } else {
+                k_ir_to32<<<g2, kT, 0, s>>>(p->d_r, m, p->d_r32, p->d_irdone);
                 RB_TRY(walk<float>(p, nc, p->d_r32, m, p->d_d32, m, s, p->d_irleft));
-            k_ir_update<<<g2, kT, 0, s>>>(p->d_d32, m, x, ldx, p->d_irdone);
+                k_ir_update<<<g2, kT, 0, s>>>(p->d_d32, m, x, ldx, p->d_irdone);
+            }
             residual(p, nc, b, ldb, x, ldx, s);
             k_ir_check<<<nblk(nc), kT, 0, s>>>(p->d_norms, p->d_norms + nc, nc,
                                                tol, p->d_irdone, p->d_iriters,
