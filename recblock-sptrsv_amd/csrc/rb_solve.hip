@@ -651,7 +651,9 @@ k_ir_residual(const int *__restrict__ rptr, const int *__restrict__ rcol,
                     const int c = c0 + q;
                     const double v = b[c * ldb + i] - acc[q];
                     r[(long)c * m + i] = v;
-                    a[q] = fmax(a[q], fabs(v));
+                    // fmax drops a NaN operand: a NaN residual counts as
+                    // +inf, so its column never passes for converged
+                    a[q] = fmax(a[q], v == v ? fabs(v) : HUGE_VAL);
                 }
         }
     }

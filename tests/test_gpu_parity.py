@@ -741,3 +741,32 @@ def test_watchdog_trip_gives_all_nan_and_check_raises(torch, schedule):
         assert residual(f, y.cpu().numpy(), b.cpu().numpy()) < 1e-12
     finally:
         q.close()
+
+
+def test_watchdog_trip_in_fp32_refinement_on_fused_schedule(torch):
+    """The FP32 refinement on the fused schedule under a forced trip (bounded
+    waits that give up after one poll; every kernel still exits normally):
+    the call reports a miss, x is all NaN, the plan's check reports the
+    watchdog, and the next call on the same plan, whose slot buffers the
+    epilogues re-armed, trips the same way instead of reading stale
+    slots."""
+    from rbsptrsv import Plan
+    from rbsptrsv._lib import RbError
+    f = gen.banded(20000, seed=4)
+    print("watchdog trip, fp32 refinement, fused", flush=True)
+    b = torch.as_tensor(gen.rhs(f.n, seed=2), device="cuda")
+    p = Plan.from_factor(f, lv=1, schedule=2, mixed=True, watchdog_spins=1)
+    try:
+        for _ in range(2):
+            with pytest.raises(RbError) as e:
+                p.solve_f32ir(b, tol=1e-12, max_iter=3)
+            assert e.value.status == 9                 # not converged
+            with pytest.raises(RbError) as e:
+                p.check()
+            assert e.value.status == 8                 # the watchdog
+            x = p.solve_f32ir(b, tol=1e-12, max_iter=1, report=False)[0]
+            with pytest.raises(RbError):
+                p.check()
+            assert bool(torch.isnan(x).all())
+    finally:
+        p.close()
