@@ -254,7 +254,8 @@ rb_status rb_plan_trace(const rb_plan *plan, unsigned long long *counters,
  * *relres the largest final max|r| / max|b|; the call synchronises on
  * stream after the second correction and each later one (once for the
  * usual two) and returns RB_ERR_NOT_CONVERGED when a column missed tol (x
- * then holds the last iterate).  With both NULL the call does not
+ * then holds the last iterate; a column whose residual holds a NaN never
+ * converges, and its relres is reported as inf).  With both NULL the call does not
  * synchronise: max_iter corrections are queued and a column that missed
  * tol sets the plan's status, which rb_plan_check reports as
  * RB_ERR_NOT_CONVERGED.  No counterpart in the reference (FP64 only). */
